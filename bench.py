@@ -1,0 +1,190 @@
+"""Benchmark of the particle-DEM step on one MI355X.
+
+Workload: 1,048,576 monodisperse spheres (r = 1/128) on a jittered cubic
+lattice in [-1,1]^3, 256^3 hash grid (cell = 2r), spring .5, damping .02,
+shear .1, boundary damping -.5, gravity -0.0003, dt 0.5.  A step is one
+ParticleSystem::update: integrate -> hash -> sort -> reorder -> collide.
+
+Prints ONE JSON line (the last line of stdout) with the throughput in
+particle-steps/s, the HBM roofline of the collide kernel (algorithmic bytes
+over its HIP-event-timed average duration) and the CPU baseline (the oracle,
+the CPU restatement of the same step, timed on this host).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s at 1M spheres, % of HBM peak; SDF-MLP vs distance-map speedup"
+HBM_PEAK_GBS = 8000.0
+N = 1 << 20
+GRID = 256
+RADIUS = 1.0 / 128.0
+DT = 0.5
+SEED = 1973
+JITTER = 0.1
+VAMP = 0.01
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cell_counts(cs: np.ndarray, grid: int) -> tuple[int, int]:
+    """C_occ (occupied cells) and C_touch (distinct cells in the 27-cell
+    neighbourhoods of occupied cells, with the grid's wrap)."""
+    occ = np.nonzero(cs != 0xFFFFFFFF)[0].astype(np.int64)
+    g = grid
+    x, y, z = occ % g, (occ // g) % g, occ // (g * g)
+    mark = np.zeros(g ** 3, bool)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                mark[(((z + dz) % g) * g + ((y + dy) % g)) * g + ((x + dx) % g)] = True
+    return int(len(occ)), int(mark.sum())
+
+
+def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
+    from fullc0ntact_amd import ParticleSystem, make_params
+
+    ps = ParticleSystem(N, (GRID, GRID, GRID))
+    ps.m_params = make_params(N, GRID, RADIUS)
+    ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
+    ps.sync()
+    log(f"[bench] warmup {warmup} steps")
+    ps.run(DT, warmup)
+    ps.sync()
+    log(f"[bench] timed {steps} steps")
+    t0 = time.perf_counter()
+    ps.run(DT, steps)
+    ps.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+
+    # Per-stage durations: HIP events around every launch on the system's
+    # stream, over a second timed region of the same workload.
+    log(f"[bench] per-stage event timing over {timing_steps} steps")
+    ctx = ps.ctx
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(timing_steps):
+        ps.update(DT)
+    stages = ctx.timing()
+    ctx.enable_timing(False)
+    ps.sync()
+    cs = ps.download("cellStart")
+    c_occ, c_touch = cell_counts(cs, GRID)
+    ps.close()
+    return dict(elapsed=elapsed, stages=stages, c_occ=c_occ, c_touch=c_touch)
+
+
+def cpu_leg(target_seconds: float, threads: int) -> dict:
+    from oracle import oracle as orc
+
+    orc.set_threads(threads)
+    P = orc.make_params(N, GRID, RADIUS)
+    pos, vel = orc.reset(N, 1, SEED, RADIUS, JITTER, VAMP)
+    st = orc.State(pos, vel, P.numCells)
+    t0 = time.perf_counter()
+    st.update(DT, P)  # warm-up step (first touch of every array)
+    first = time.perf_counter() - t0
+    k = int(max(1, min(20, target_seconds / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    st.run(DT, k, P)
+    el = time.perf_counter() - t0
+    orc.set_threads(1)
+    return dict(value=N * k / el, steps=k, seconds=el)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported "
+              "(this build has no multi-GPU path).", file=sys.stderr)
+        return 2
+
+    g = gpu_leg(a.steps, a.warmup, a.timing_steps)
+    ms = g["elapsed"] / a.steps * 1e3
+    value = N * a.steps / g["elapsed"]
+    st = g["stages"]
+    stage_us = {k: round(v[1] / max(v[0], 1) * 1e3, 3) for k, v in st.items()}
+    for k, v in stage_us.items():
+        log(f"[bench] stage {k:16s} {v:10.2f} us")
+
+    collide_us = stage_us.get("collide")
+    collide_bytes = 52 * N + 8 * g["c_touch"]
+    roof = None
+    if collide_us:
+        achieved = collide_bytes / (collide_us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "collide", "bytes_per_launch": collide_bytes,
+                "avg_launch_us": collide_us}
+
+    cpu = None
+    if not a.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log(f"[bench] CPU baseline with {threads} threads")
+        c = cpu_leg(a.cpu_seconds, threads)
+        cpu = {"value": round(c["value"], 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{c['steps']} updates of the same 1,048,576-sphere lattice after one "
+                         f"warm-up update, {threads} OpenMP threads on {cpu_model()} "
+                         f"({c['seconds']:.1f} s)"}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "dem-1M-spheres", "particles": N, "grid": [GRID] * 3,
+                   "radius": RADIUS, "dt": DT, "seed": SEED, "C_occ": g["c_occ"],
+                   "C_touch": g["c_touch"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages_us": stage_us,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

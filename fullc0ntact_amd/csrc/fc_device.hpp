@@ -1,0 +1,110 @@
+// Device-side arithmetic of the particle step, shared by the HIP kernels.
+//
+// Every function keeps the exact IEEE operation order of the reference
+// kernels so that, compiled without FMA contraction, the GPU reproduces the
+// CPU restatement bit for bit.  Vector arithmetic is spelled out per
+// component on a plain struct instead of HIP's vector operators, so the
+// order of every add and multiply is visible here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "fc_dem.h"
+
+namespace fc {
+
+struct v3f {
+  float x, y, z;
+};
+
+__device__ __forceinline__ v3f make_v3(float x, float y, float z) { return v3f{x, y, z}; }
+__device__ __forceinline__ v3f xyz(const float4& a) { return v3f{a.x, a.y, a.z}; }
+
+// dot(a,b) = a.x*b.x + a.y*b.y + a.z*b.z, evaluated left to right
+// (cutil_math.cuh:379-382).
+__device__ __forceinline__ float dot3(v3f a, v3f b) {
+  float t = a.x * b.x;
+  t = t + a.y * b.y;
+  return t + a.z * b.z;
+}
+
+// Cell coordinate along one axis: (int)floor((p - origin) / cellSize)
+// (calcGridPos, particles_kernel.cu:78-85).  The float quotient is
+// converted with saturation to [-2^31, 2^31-1] and NaN maps to 0, which
+// makes out-of-range and non-finite positions deterministic (the reference
+// leaves them undefined).
+__device__ __forceinline__ int cell_coord(float p, float origin, float cell) {
+  float q = floorf((p - origin) / cell);
+  if (!(q == q)) return 0;
+  if (q >= 2147483648.0f) return 2147483647;
+  if (q < -2147483648.0f) return (int)0x80000000;
+  return (int)q;
+}
+
+// Linear cell id with power-of-two wrap (calcGridHash,
+// particles_kernel.cu:88-94).  The reference multiplies with __umul24; the
+// operands are below 2^24 for every grid with x*y < 2^24, where the 32-bit
+// product is identical.
+__device__ __forceinline__ uint32_t grid_hash(int gx, int gy, int gz, const fc_uint3& G) {
+  uint32_t x = (uint32_t)gx & (G.x - 1u);
+  uint32_t y = (uint32_t)gy & (G.y - 1u);
+  uint32_t z = (uint32_t)gz & (G.z - 1u);
+  return (z * G.y) * G.x + y * G.x + x;
+}
+
+__device__ __forceinline__ uint32_t hash_of(const fc_sim_params& P, float px, float py, float pz) {
+  int gx = cell_coord(px, P.worldOrigin.x, P.cellSize.x);
+  int gy = cell_coord(py, P.worldOrigin.y, P.cellSize.y);
+  int gz = cell_coord(pz, P.worldOrigin.z, P.cellSize.z);
+  return grid_hash(gx, gy, gz, P.gridSize);
+}
+
+// Pair force of sphere B on sphere A (collideSpheres,
+// particles_kernel.cu:185-218):
+//   d = pB - pA; dist = sqrt(d.d); R = rA + rB
+//   dist < R:  n = d * (1/dist)               (float3 / float, cutil_math.cuh:345-349)
+//              u = vB - vA;  ut = u - (u.n) n
+//              f = (-k (R - dist)) n + c u + s ut + a d
+//   else 0.
+__device__ __forceinline__ v3f collide_spheres(v3f pA, v3f pB, v3f vA, v3f vB, float rA, float rB,
+                                               float spring, float damping, float shear,
+                                               float attraction) {
+  v3f rel = make_v3(pB.x - pA.x, pB.y - pA.y, pB.z - pA.z);
+  float dist = sqrtf(dot3(rel, rel));
+  float collideDist = rA + rB;
+  v3f f = make_v3(0.0f, 0.0f, 0.0f);
+  if (dist < collideDist) {
+    float inv = 1.0f / dist;
+    v3f nrm = make_v3(rel.x * inv, rel.y * inv, rel.z * inv);
+    v3f rv = make_v3(vB.x - vA.x, vB.y - vA.y, vB.z - vA.z);
+    float rvn = dot3(rv, nrm);
+    v3f tv = make_v3(rv.x - nrm.x * rvn, rv.y - nrm.y * rvn, rv.z - nrm.z * rvn);
+    float s = -spring * (collideDist - dist);
+    f = make_v3(nrm.x * s, nrm.y * s, nrm.z * s);
+    f = make_v3(f.x + rv.x * damping, f.y + rv.y * damping, f.z + rv.z * damping);
+    f = make_v3(f.x + tv.x * shear, f.y + tv.y * shear, f.z + tv.z * shear);
+    f = make_v3(f.x + rel.x * attraction, f.y + rel.y * attraction, f.z + rel.z * attraction);
+  }
+  return f;
+}
+
+// integrate_functor body (particles_kernel.cu:46-74): semi-implicit Euler
+// and the box walls at +-1 (not derived from worldOrigin), tested in the
+// order x+, x-, y+, z+, z-, y-.
+__device__ __forceinline__ void integrate_box(const fc_sim_params& P, float dt, v3f& pos, v3f& vel) {
+  vel = make_v3(vel.x + P.gravity.x * dt, vel.y + P.gravity.y * dt, vel.z + P.gravity.z * dt);
+  vel = make_v3(vel.x * P.globalDamping, vel.y * P.globalDamping, vel.z * P.globalDamping);
+  pos = make_v3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
+  const float r = P.particleRadius;
+  const float hi = 1.0f - r;
+  const float lo = -1.0f + r;
+  const float bd = P.boundaryDamping;
+  if (pos.x > hi) { pos.x = hi; vel.x *= bd; }
+  if (pos.x < lo) { pos.x = lo; vel.x *= bd; }
+  if (pos.y > hi) { pos.y = hi; vel.y *= bd; }
+  if (pos.z > hi) { pos.z = hi; vel.z *= bd; }
+  if (pos.z < lo) { pos.z = lo; vel.z *= bd; }
+  if (pos.y < lo) { pos.y = lo; vel.y *= bd; }
+}
+
+}  // namespace fc

@@ -1,0 +1,181 @@
+// Internal declarations shared by the HIP translation units of libfc_dem.so.
+// Not part of the public ABI (that is include/fc_dem.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "fc_dem.h"
+
+struct fc_context;
+
+namespace fc {
+
+// ---------------------------------------------------------------------------
+// Errors.  Every entry point returns an fc_status and leaves a thread-local
+// message readable through fc_last_error().
+// ---------------------------------------------------------------------------
+fc_status set_error(fc_status s, const std::string& msg);
+fc_status hip_error(hipError_t e, const char* what);
+
+#define FC_HIP_TRY(call)                                   \
+  do {                                                     \
+    hipError_t fc_e_ = (call);                             \
+    if (fc_e_ != hipSuccess) return ::fc::hip_error(fc_e_, #call); \
+  } while (0)
+
+#define FC_TRY(call)                                       \
+  do {                                                     \
+    fc_status fc_s_ = (call);                              \
+    if (fc_s_ != FC_SUCCESS) return fc_s_;                 \
+  } while (0)
+
+// Bits of the device-side error word (ctx->devFlags).
+enum : uint32_t {
+  DEVERR_SORT_STALLED = 1u << 0,   // look-back exceeded its spin bound
+  DEVERR_SORT_KEY_RANGE = 1u << 1, // a key had bits above the sorted width
+};
+
+constexpr uint32_t CELL_EMPTY = 0xffffffffu;
+
+// ---------------------------------------------------------------------------
+// Radix sort (radix_sort.hip).
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;               // 4 wave64s
+constexpr int RS_ITEMS = 16;                  // keys per thread per tile
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS; // 4096 keys
+constexpr int RS_RADIX_BITS = 8;
+constexpr int RS_RADIX = 1 << RS_RADIX_BITS;
+constexpr int RS_MAX_PASSES = 4;
+constexpr int RS_MAX_HIST_BLOCKS = 64;
+
+// Device counters used by the sort; all self-resetting (see radix_sort.hip).
+enum : int {
+  RSC_HIST_DONE = 0,
+  RSC_TILE0 = 1,      // RSC_TILE0 + pass: tile-id ticket of each pass
+  RSC_EPOCH = 8,      // bumped once per sort by the histogram kernel
+  RSC_COUNT = 16
+};
+
+struct SortScratch {
+  uint32_t* keysA = nullptr;
+  uint32_t* valsA = nullptr;
+  uint32_t* keysB = nullptr;
+  uint32_t* valsB = nullptr;
+  size_t capacity = 0;               // elements in keysA/valsA/keysB/valsB
+  uint64_t* status = nullptr;        // tile x digit look-back words
+  size_t statusCapacity = 0;         // in words
+  uint32_t* partial = nullptr;       // [RS_MAX_HIST_BLOCKS][RS_MAX_PASSES][RS_RADIX]
+  uint32_t* offsets = nullptr;       // [RS_MAX_PASSES][RS_RADIX] exclusive bucket starts
+  uint32_t* counters = nullptr;      // [RSC_COUNT]
+};
+
+struct StageTimer;
+
+// Number of significant bits needed for keys in [0, numCells).
+int key_bits_for_cells(uint32_t numCells);
+
+// Stable ascending LSD sort of (key, value) pairs on the low `bits` bits.
+// valsIn == nullptr means the values are the identity 0..n-1.  keysIn may
+// equal keysOut and valsIn may equal valsOut (in-place); intermediate passes
+// use the scratch buffers.  All launches go to `stream`, nothing is
+// allocated, nothing synchronises: the call is graph-capturable.
+hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
+                            const uint32_t* keysIn, const uint32_t* valsIn,
+                            uint32_t* keysOut, uint32_t* valsOut, uint32_t n, int bits,
+                            StageTimer* timer);
+
+// ---------------------------------------------------------------------------
+// Per-stage timing with HIP events.
+// ---------------------------------------------------------------------------
+struct StageTimer {
+  struct Pending { int stage; hipEvent_t begin, end; };
+  struct Stage { std::string name; uint64_t launches = 0; double total_ms = 0.0; };
+  bool enabled = false;
+  std::vector<Stage> stages;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  int stage_id(const char* name);
+  hipEvent_t get_event();
+  // begin/end bracket one launch on `stream`; no-ops when disabled.
+  void begin(hipStream_t stream, const char* name);
+  void end(hipStream_t stream);
+  hipError_t flush();  // synchronises pending events and accumulates
+  void reset();
+  void destroy();
+ private:
+  int open_stage = -1;
+  hipEvent_t open_event = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// DEM stage launchers (dem_kernels.hip).  Parameters are passed by value as
+// kernel arguments (no constant-memory upload per step).
+// ---------------------------------------------------------------------------
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
+hipError_t launch_integrate(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+                            float dt, uint32_t n);
+hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* hash,
+                            uint32_t* index, const float4* pos, uint32_t n);
+// integrate + calcHash fused, plus clearing of the cells listed in the
+// previous step's sorted hashes (prevSortedHash may be null: no clearing).
+hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+                                 float dt, uint32_t* keysOut, const uint32_t* prevSortedHash,
+                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n);
+hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
+                          float4* sortedVel, const uint32_t* hash, const uint32_t* index,
+                          const float4* oldPos, const float4* oldVel, uint32_t n,
+                          uint32_t numCells);
+hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
+                          const float4* sortedPos, const float4* sortedVel,
+                          const uint32_t* index, const uint32_t* cellStart,
+                          const uint32_t* cellEnd, uint32_t n);
+
+fc_status validate_params(const fc_sim_params& P);
+// Grows the context's sort scratch to hold n keys (synchronises first).
+fc_status ensure_sort_capacity(fc_context* ctx, size_t n);
+// Synchronises the context stream and converts the device error word.
+fc_status check_device_flags(fc_context* ctx);
+fc_sim_params default_params(uint32_t numParticles, fc_uint3 gridSize);
+
+}  // namespace fc
+
+// ---------------------------------------------------------------------------
+// Opaque handles.
+// ---------------------------------------------------------------------------
+struct fc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fc_sim_params params{};
+  bool paramsSet = false;
+  fc::SortScratch sort;
+  uint32_t* devFlags = nullptr;   // device error word
+  fc::StageTimer timer;
+};
+
+struct fc_system {
+  fc_context* ctx = nullptr;
+  uint32_t n = 0;
+  fc_sim_params params{};
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  float4* sortedPos = nullptr;
+  float4* sortedVel = nullptr;
+  uint32_t* hash = nullptr;       // sorted hashes of the last update
+  uint32_t* index = nullptr;      // sorted original indices of the last update
+  uint32_t* keys = nullptr;       // unsorted hashes written by integrate+hash
+  uint32_t* cellStart = nullptr;
+  uint32_t* cellEnd = nullptr;
+  // True when the cell tables hold exactly the cells listed in `hash`, so the
+  // next update may clear only those instead of the whole table.
+  bool tablesTrackHash = false;
+  // Captured steady-state update (graph-launched by fc_system_run).
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  float graphDt = 0.0f;
+  fc_sim_params graphParams{};
+};

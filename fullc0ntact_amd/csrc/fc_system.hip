@@ -1,0 +1,343 @@
+// The particle-system object behind fc_system_* (ParticleSystem,
+// particleSystem.h:22-136 / particleSystem.cpp:30-509): buffer ownership,
+// initial configurations, update(dt) and graph-launched multi-step runs.
+#include "fc_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace fc {
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014).  Portable across compilers and
+// hosts, unlike srand/rand (particleSystem.cpp:375-383).
+struct SplitMix64 {
+  uint64_t state;
+  explicit SplitMix64(uint64_t seed) : state(seed) {}
+  uint64_t next() {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  // 24 random bits scaled to [0, 1): exactly representable floats.
+  float uniform01() { return (float)(next() >> 40) * (1.0f / 16777216.0f); }
+  // U(-1, 1) as 2u - 1 (the reference's frand()*2-1).
+  float symmetric() { return uniform01() * 2.0f - 1.0f; }
+};
+
+uint32_t lattice_side(uint32_t n) {
+  uint32_t s = 0;
+  while ((uint64_t)s * s * s < n) ++s;
+  return s;
+}
+
+void free_system_buffers(fc_system* sys) {
+  hipFree(sys->pos);
+  hipFree(sys->vel);
+  hipFree(sys->sortedPos);
+  hipFree(sys->sortedVel);
+  hipFree(sys->hash);
+  hipFree(sys->index);
+  hipFree(sys->keys);
+  hipFree(sys->cellStart);
+  hipFree(sys->cellEnd);
+  sys->pos = sys->vel = sys->sortedPos = sys->sortedVel = nullptr;
+  sys->hash = sys->index = sys->keys = sys->cellStart = sys->cellEnd = nullptr;
+}
+
+void drop_graph(fc_system* sys) {
+  if (sys->graphExec) hipGraphExecDestroy(sys->graphExec);
+  if (sys->graph) hipGraphDestroy(sys->graph);
+  sys->graphExec = nullptr;
+  sys->graph = nullptr;
+}
+
+fc_status alloc_tables(fc_system* sys, uint32_t numCells) {
+  hipFree(sys->cellStart);
+  hipFree(sys->cellEnd);
+  sys->cellStart = sys->cellEnd = nullptr;
+  FC_HIP_TRY(hipMalloc(&sys->cellStart, (size_t)numCells * sizeof(uint32_t)));
+  FC_HIP_TRY(hipMalloc(&sys->cellEnd, (size_t)numCells * sizeof(uint32_t)));
+  sys->tablesTrackHash = false;
+  return FC_SUCCESS;
+}
+
+// One update(dt) (particleSystem.cpp:222-284): integrate -> calcHash ->
+// sort -> reorder/findCellStart -> collide, enqueued on the context stream.
+// Nothing here allocates or synchronises, so it can be stream-captured.
+fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
+  fc_context* ctx = sys->ctx;
+  hipStream_t st = ctx->stream;
+  const fc_sim_params& P = sys->params;
+  const uint32_t n = sys->n;
+  hipError_t e = hipSuccess;
+  const uint32_t* prev = nullptr;
+  if (sys->tablesTrackHash) {
+    prev = sys->hash;
+  } else {
+    if (timer) timer->begin(st, "clear_tables");
+    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells);
+    if (e == hipSuccess) e = launch_fill_u32(st, sys->cellEnd, CELL_EMPTY, P.numCells);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "clear cell tables");
+  }
+  if (n == 0) {
+    sys->tablesTrackHash = false;
+    return FC_SUCCESS;
+  }
+  if (timer) timer->begin(st, "integrate_hash");
+  e = launch_integrate_hash(st, P, sys->pos, sys->vel, dt, sys->keys, prev, sys->cellStart,
+                            sys->cellEnd, n);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "integrate_hash");
+  e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index, n,
+                       key_bits_for_cells(P.numCells), timer);
+  if (e != hipSuccess) return hip_error(e, "sort");
+  if (timer) timer->begin(st, "reorder");
+  e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,
+                     sys->index, sys->pos, sys->vel, n, P.numCells);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "reorder");
+  if (timer) timer->begin(st, "collide");
+  e = launch_collide(st, P, sys->vel, sys->sortedPos, sys->sortedVel, sys->index, sys->cellStart,
+                     sys->cellEnd, n);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "collide");
+  sys->tablesTrackHash = true;
+  return FC_SUCCESS;
+}
+
+fc_status capture_graph(fc_system* sys, float dt) {
+  drop_graph(sys);
+  hipStream_t st = sys->ctx->stream;
+  FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  fc_status s = enqueue_update(sys, dt, nullptr);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != FC_SUCCESS) {
+    if (g) hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  sys->graph = g;
+  e = hipGraphInstantiate(&sys->graphExec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(sys);
+    return hip_error(e, "hipGraphInstantiate");
+  }
+  sys->graphDt = dt;
+  sys->graphParams = sys->params;
+  return FC_SUCCESS;
+}
+
+}  // namespace
+
+}  // namespace fc
+
+using namespace fc;
+
+extern "C" {
+
+fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 gridSize,
+                           fc_system** out) {
+  if (!ctx || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *out = nullptr;
+  fc_sim_params P = default_params(numParticles, gridSize);
+  uint64_t cells = (uint64_t)gridSize.x * gridSize.y * gridSize.z;
+  if (cells > 0xffffffffull) return set_error(FC_ERROR_CAPACITY, "grid has 2^32 or more cells");
+  FC_TRY(validate_params(P));
+  fc_system* sys = new fc_system();
+  sys->ctx = ctx;
+  sys->n = numParticles;
+  sys->params = P;
+  const size_t n = numParticles ? numParticles : 1;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&sys->pos, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->vel, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->sortedPos, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->sortedVel, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->hash, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&sys->index, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&sys->keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(sys->pos, 0, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMemset(sys->vel, 0, n * sizeof(float4));
+  if (e != hipSuccess) {
+    free_system_buffers(sys);
+    delete sys;
+    return hip_error(e, "fc_system_create allocation");
+  }
+  fc_status s = alloc_tables(sys, P.numCells);
+  // The sort's scratch is sized here so that updates never allocate.
+  if (s == FC_SUCCESS) s = ensure_sort_capacity(ctx, n);
+  if (s != FC_SUCCESS) {
+    free_system_buffers(sys);
+    delete sys;
+    return s;
+  }
+  *out = sys;
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_destroy(fc_system* sys) {
+  if (!sys) return FC_SUCCESS;
+  if (sys->ctx && sys->ctx->stream) hipStreamSynchronize(sys->ctx->stream);
+  drop_graph(sys);
+  free_system_buffers(sys);
+  delete sys;
+  return FC_SUCCESS;
+}
+
+uint32_t fc_system_num_particles(const fc_system* sys) { return sys ? sys->n : 0u; }
+
+fc_status fc_system_set_params(fc_system* sys, const fc_sim_params* params) {
+  if (!sys || !params) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  FC_TRY(validate_params(*params));
+  if (params->numCells != sys->params.numCells) {
+    FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+    drop_graph(sys);
+    FC_TRY(alloc_tables(sys, params->numCells));
+  }
+  sys->params = *params;
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_get_params(const fc_system* sys, fc_sim_params* params) {
+  if (!sys || !params) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *params = sys->params;
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_set_array(fc_system* sys, fc_particle_array array, const float* data, int start,
+                              int count) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  if (array != FC_POSITION && array != FC_VELOCITY)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle array");
+  if (start < 0 || count < 0 || (uint64_t)start + (uint64_t)count > sys->n)
+    return set_error(FC_ERROR_INVALID_VALUE, "start/count outside [0, numParticles)");
+  if (count == 0) return FC_SUCCESS;
+  if (!data) return set_error(FC_ERROR_INVALID_VALUE, "data is NULL");
+  float4* dst = array == FC_POSITION ? sys->pos : sys->vel;
+  FC_HIP_TRY(hipMemcpyAsync(dst + start, data, (size_t)count * sizeof(float4),
+                            hipMemcpyHostToDevice, sys->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_get_array(fc_system* sys, fc_particle_array array, float* out) {
+  if (!sys || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (array != FC_POSITION && array != FC_VELOCITY)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle array");
+  if (sys->n == 0) return FC_SUCCESS;
+  const float4* src = array == FC_POSITION ? sys->pos : sys->vel;
+  FC_HIP_TRY(hipMemcpyAsync(out, src, (size_t)sys->n * sizeof(float4), hipMemcpyDeviceToHost,
+                            sys->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  return check_device_flags(sys->ctx);
+}
+
+fc_status fc_system_reset(fc_system* sys, fc_particle_config config, uint64_t seed,
+                          float jitterFrac, float velAmp) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  if (config != FC_CONFIG_GRID && config != FC_CONFIG_RANDOM)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle config");
+  const uint32_t n = sys->n;
+  if (n == 0) return FC_SUCCESS;
+  std::vector<float> pos((size_t)n * 4), vel((size_t)n * 4);
+  SplitMix64 rng(seed);
+  const float r = sys->params.particleRadius;
+  if (config == FC_CONFIG_GRID) {
+    const uint32_t s = lattice_side(n);
+    const float spacing = r * 2.0f;
+    const float jitter = r * jitterFrac;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t x = i % s, y = (i / s) % s, z = i / (s * s);
+      const uint32_t c[3] = {x, y, z};
+      for (int a = 0; a < 3; ++a)
+        pos[(size_t)i * 4 + a] = ((spacing * (float)c[a] + r) - 1.0f) + rng.symmetric() * jitter;
+      pos[(size_t)i * 4 + 3] = 1.0f;
+      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
+      vel[(size_t)i * 4 + 3] = 0.0f;
+    }
+  } else {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int a = 0; a < 3; ++a) pos[(size_t)i * 4 + a] = 2.0f * (rng.uniform01() - 0.5f);
+      pos[(size_t)i * 4 + 3] = 1.0f;
+      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
+      vel[(size_t)i * 4 + 3] = 0.0f;
+    }
+  }
+  FC_TRY(fc_system_set_array(sys, FC_POSITION, pos.data(), 0, (int)n));
+  FC_TRY(fc_system_set_array(sys, FC_VELOCITY, vel.data(), 0, (int)n));
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_update(fc_system* sys, float deltaTime) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  StageTimer* t = sys->ctx->timer.enabled ? &sys->ctx->timer : nullptr;
+  return enqueue_update(sys, deltaTime, t);
+}
+
+fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  if (steps == 0) return FC_SUCCESS;
+  fc_context* ctx = sys->ctx;
+  if (ctx->timer.enabled || sys->n == 0) {
+    for (uint32_t i = 0; i < steps; ++i) FC_TRY(fc_system_update(sys, deltaTime));
+    return FC_SUCCESS;
+  }
+  if (!sys->tablesTrackHash) {
+    FC_TRY(enqueue_update(sys, deltaTime, nullptr));
+    --steps;
+  }
+  if (steps == 0) return FC_SUCCESS;
+  if (!sys->graphExec || sys->graphDt != deltaTime ||
+      std::memcmp(&sys->graphParams, &sys->params, sizeof(fc_sim_params)) != 0) {
+    FC_TRY(capture_graph(sys, deltaTime));
+  }
+  for (uint32_t i = 0; i < steps; ++i)
+    FC_HIP_TRY(hipGraphLaunch(sys->graphExec, ctx->stream));
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_device_arrays(fc_system* sys, float** pos, float** vel, float** sortedPos,
+                                  float** sortedVel, uint32_t** hash, uint32_t** index,
+                                  uint32_t** cellStart, uint32_t** cellEnd) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  if (pos) *pos = (float*)sys->pos;
+  if (vel) *vel = (float*)sys->vel;
+  if (sortedPos) *sortedPos = (float*)sys->sortedPos;
+  if (sortedVel) *sortedVel = (float*)sys->sortedVel;
+  if (hash) *hash = sys->hash;
+  if (index) *index = sys->index;
+  if (cellStart) *cellStart = sys->cellStart;
+  if (cellEnd) *cellEnd = sys->cellEnd;
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_dump_grid(fc_system* sys, uint32_t* maxParticlesPerCell,
+                              uint32_t* occupiedCells) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  const uint32_t C = sys->params.numCells;
+  std::vector<uint32_t> cs(C), ce(C);
+  FC_HIP_TRY(hipMemcpyAsync(cs.data(), sys->cellStart, (size_t)C * 4, hipMemcpyDeviceToHost,
+                            sys->ctx->stream));
+  FC_HIP_TRY(hipMemcpyAsync(ce.data(), sys->cellEnd, (size_t)C * 4, hipMemcpyDeviceToHost,
+                            sys->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  uint32_t mx = 0, occ = 0;
+  for (uint32_t i = 0; i < C; ++i) {
+    if (cs[i] != CELL_EMPTY) {
+      ++occ;
+      uint32_t c = ce[i] - cs[i];
+      if (c > mx) mx = c;
+    }
+  }
+  if (maxParticlesPerCell) *maxParticlesPerCell = mx;
+  if (occupiedCells) *occupiedCells = occ;
+  return FC_SUCCESS;
+}
+
+}  // extern "C"

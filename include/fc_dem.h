@@ -1,0 +1,229 @@
+/*
+ * fc_dem.h - C ABI of the MI355X (gfx950) particle-DEM step.
+ *
+ * This is the drop-in boundary for the particle step of FullC0ntact's GPU
+ * particle system.  Every entry point names the reference interface it
+ * replaces (paths relative to the FullC0ntact tree).  All functions are
+ * plain C: raw pointers and sizes, no C++ or torch types.  Device pointers
+ * are HIP device allocations (fc_allocate_array or any hipMalloc'd buffer);
+ * host pointers are ordinary host memory.
+ *
+ * Differences from the reference ABI, all deliberate:
+ *   - Every call returns an fc_status instead of calling exit() on error
+ *     (reference: cutilSafeCall, particleSystem.cu:67,180).
+ *   - Parameters live in an explicit fc_context (one parameter set and one
+ *     HIP stream per context) instead of the process-global __constant__
+ *     SimParams (particles_kernel.cu:36).  A binding that wants the
+ *     reference's global style keeps one context in a static.
+ *   - Launches are asynchronous on the context's stream; copies to/from the
+ *     host synchronise that stream.
+ */
+#ifndef FC_DEM_H
+#define FC_DEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FC_DEM_ABI_VERSION 1
+
+/* 12-byte vectors, unpadded, matching CUDA's float3/uint3 used inside
+ * SimParams (particles_kernel.cuh:27-48). */
+typedef struct fc_float3 { float x, y, z; } fc_float3;
+typedef struct fc_uint3 { uint32_t x, y, z; } fc_uint3;
+
+/* Same fields, order and 104-byte layout as SimParams
+ * (cuda_addon/particles_kernel.cuh:27-48); host mirror of
+ * SimulationParameters<T> (inshape3dcore/grid/uniformgrid.h:470-493). */
+typedef struct fc_sim_params {
+  fc_float3 colliderPos;
+  float colliderRadius;
+  fc_float3 gravity;
+  float globalDamping;
+  float particleRadius;
+  fc_uint3 gridSize;
+  uint32_t numCells;
+  fc_float3 worldOrigin;
+  fc_float3 cellSize;
+  uint32_t numBodies;
+  uint32_t maxParticlesPerCell;
+  float spring;
+  float damping;
+  float shear;
+  float attraction;
+  float boundaryDamping;
+} fc_sim_params;
+
+typedef enum fc_status {
+  FC_SUCCESS = 0,
+  FC_ERROR_INVALID_VALUE = 1,    /* null pointer, zero-size grid, bad enum   */
+  FC_ERROR_GRID_NOT_POW2 = 2,    /* hash wraps with & (G-1): G must be 2^k   */
+  FC_ERROR_HIP = 3,              /* a HIP runtime call failed                */
+  FC_ERROR_OUT_OF_MEMORY = 4,
+  FC_ERROR_NO_DEVICE = 5,
+  FC_ERROR_SORT_STALLED = 6,     /* radix-sort look-back hit its spin bound  */
+  FC_ERROR_CAPACITY = 7,         /* size beyond a documented maximum         */
+  FC_ERROR_NOT_SUPPORTED = 8
+} fc_status;
+
+/* Particle arrays (ParticleSystem::ParticleArray, particleSystem.h:35-39). */
+typedef enum fc_particle_array { FC_POSITION = 0, FC_VELOCITY = 1 } fc_particle_array;
+
+/* Initial configurations (ParticleSystem::ParticleConfig,
+ * particleSystem.h:28-33). */
+typedef enum fc_particle_config { FC_CONFIG_RANDOM = 0, FC_CONFIG_GRID = 1 } fc_particle_config;
+
+#define FC_CELL_EMPTY 0xffffffffu
+
+typedef struct fc_context fc_context;
+typedef struct fc_system fc_system;
+
+/* ------------------------------------------------------------------------- */
+/* Runtime, errors, memory.                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces cudaInit(argc, argv) (particleSystem.cuh:14): selects a device and
+ * creates a context with its own HIP stream and the ParticleSystem default
+ * parameters (particleSystem.cpp:48-68).  device < 0 keeps the current one. */
+fc_status fc_context_create(int device, fc_context** out);
+fc_status fc_context_destroy(fc_context* ctx);
+/* Text of the last error raised on this thread (never NULL). */
+const char* fc_last_error(void);
+int fc_abi_version(void);
+/* Number of HIP devices visible (0 when none), without creating a context. */
+int fc_device_count(void);
+
+/* allocateArray / freeArray (particleSystem.cuh:16-17; the reference declares
+ * the size as int but defines it as size_t, particleSystem.cu:65). */
+fc_status fc_allocate_array(void** devPtr, size_t size);
+fc_status fc_free_array(void* devPtr);
+/* threadSync (particleSystem.cuh:19): waits for the context's stream and
+ * reports any deferred device-side error (e.g. FC_ERROR_SORT_STALLED). */
+fc_status fc_thread_sync(fc_context* ctx);
+/* copyArrayToDevice(device, host, offset, size) (particleSystem.cuh:22). */
+fc_status fc_copy_array_to_device(fc_context* ctx, void* device, const void* host, size_t offset, size_t size);
+/* copyArrayFromDevice (particleSystem.cuh:21) without the GL resource: copies
+ * size bytes starting at byte offset of the device buffer. */
+fc_status fc_copy_array_from_device(fc_context* ctx, void* host, const void* device, size_t offset, size_t size);
+
+/* ------------------------------------------------------------------------- */
+/* Per-stage kernels on caller-owned device arrays (particleSystem.cuh:29-61) */
+/* ------------------------------------------------------------------------- */
+
+/* setParameters(SimParams*) (particleSystem.cuh:29).  Validates that every
+ * grid dimension is a power of two and numCells == x*y*z. */
+fc_status fc_set_parameters(fc_context* ctx, const fc_sim_params* params);
+fc_status fc_get_parameters(const fc_context* ctx, fc_sim_params* params);
+
+/* integrateSystem(pos, vel, dt, n) (particleSystem.cuh:31-34; kernel
+ * integrate_functor, particles_kernel.cu:39-75): box walls at +-1. */
+fc_status fc_integrate_system(fc_context* ctx, float* pos, float* vel, float deltaTime, uint32_t numParticles);
+
+/* calcHash(hash, index, pos, n) (particleSystem.cuh:36-39; calcHashD,
+ * particles_kernel.cu:97-115). */
+fc_status fc_calc_hash(fc_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                       const float* pos, uint32_t numParticles);
+
+/* sortParticles(hash, index, n) (particleSystem.cuh:61; thrust::sort_by_key
+ * at particleSystem.cu:265-270): stable ascending sort of the (hash, index)
+ * pairs in place, by an LSD onesweep radix sort over the significant bits of
+ * numCells-1 taken from the context parameters. */
+fc_status fc_sort_particles(fc_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
+                            uint32_t numParticles);
+
+/* reorderDataAndFindCellStart (particleSystem.cuh:41-50; kernel
+ * particles_kernel.cu:119-182).  Clears cellStart AND cellEnd to
+ * FC_CELL_EMPTY first (as particledem.cu:229-230 does) so every table entry
+ * is defined. */
+fc_status fc_reorder_data_and_find_cell_start(fc_context* ctx, uint32_t* cellStart, uint32_t* cellEnd,
+                                              float* sortedPos, float* sortedVel,
+                                              const uint32_t* gridParticleHash,
+                                              const uint32_t* gridParticleIndex,
+                                              const float* oldPos, const float* oldVel,
+                                              uint32_t numParticles, uint32_t numCells);
+
+/* collide (particleSystem.cuh:52-59; collideD, particles_kernel.cu:256-293):
+ * 27-cell spring-dashpot-shear force plus the collider sphere, written as
+ * newVel[index[k]] = (vel + force, 0). */
+fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, const float* sortedVel,
+                     const uint32_t* gridParticleIndex, const uint32_t* cellStart,
+                     const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells);
+
+/* ------------------------------------------------------------------------- */
+/* Particle system (ParticleSystem class, particleSystem.h:22-136).          */
+/* The system owns every device buffer; update() runs the reference order     */
+/* integrate -> calcHash -> sort -> reorder -> collide                        */
+/* (particleSystem.cpp:222-284).                                              */
+/* ------------------------------------------------------------------------- */
+
+/* ParticleSystem(numParticles, gridSize, false) (particleSystem.cpp:30-72):
+ * the system starts with the constructor's default parameters (radius 1/16,
+ * cell 2r, origin (-1,-1,-1), spring .5, damping .02, shear .1, ...) and all
+ * particles at the origin with zero velocity. */
+fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 gridSize, fc_system** out);
+fc_status fc_system_destroy(fc_system* sys);
+uint32_t fc_system_num_particles(const fc_system* sys);
+
+/* m_params writes/reads (the set* setters of particleSystem.h:64-72 are
+ * field writes on a copy; see fc_particle_system.hpp). */
+fc_status fc_system_set_params(fc_system* sys, const fc_sim_params* params);
+fc_status fc_system_get_params(const fc_system* sys, fc_sim_params* params);
+
+/* setArray(array, data, start, count) (particleSystem.cpp:350-373) for both
+ * POSITION and VELOCITY (the reference uploads nothing for POSITION in
+ * non-GL mode; here both upload). data holds count float4 records. */
+fc_status fc_system_set_array(fc_system* sys, fc_particle_array array, const float* data, int start, int count);
+/* getArray(array) (particleSystem.cpp:323-348): copies all numParticles
+ * float4 records into out (4*numParticles floats). */
+fc_status fc_system_get_array(fc_system* sys, fc_particle_array array, float* out);
+
+/* reset(config) (particleSystem.cpp:404-444) with a portable SplitMix64
+ * stream instead of srand/rand: CONFIG_GRID puts particles on the
+ * s x s x s lattice (s = smallest integer with s^3 >= n) of spacing 2r from
+ * -1+r, jittered by U(-jitterFrac*r, jitterFrac*r) per axis, velocities
+ * U(-velAmp, velAmp); CONFIG_RANDOM draws positions U(-1,1)^3 and
+ * velocities U(-velAmp, velAmp).  pos.w = 1, vel.w = 0. */
+fc_status fc_system_reset(fc_system* sys, fc_particle_config config, uint64_t seed, float jitterFrac, float velAmp);
+
+/* update(deltaTime) (particleSystem.cpp:222-284). */
+fc_status fc_system_update(fc_system* sys, float deltaTime);
+/* steps consecutive update(deltaTime) calls.  Results are bit-identical to
+ * calling fc_system_update steps times. */
+fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps);
+
+/* Device pointers of the system's buffers (for zero-copy consumers).  Any
+ * pointer argument may be NULL.  float arrays are float4[numParticles]. */
+fc_status fc_system_device_arrays(fc_system* sys, float** pos, float** vel, float** sortedPos,
+                                  float** sortedVel, uint32_t** hash, uint32_t** index,
+                                  uint32_t** cellStart, uint32_t** cellEnd);
+
+/* dumpGrid (particleSystem.cpp:286-301) without the printf: the largest
+ * cellEnd-cellStart over non-empty cells, plus the number of occupied cells. */
+fc_status fc_system_dump_grid(fc_system* sys, uint32_t* maxParticlesPerCell, uint32_t* occupiedCells);
+
+/* ------------------------------------------------------------------------- */
+/* Per-stage timing with HIP events on the context stream.                   */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fc_stage_timing {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} fc_stage_timing;
+
+/* When enabled, every stage launched by fc_system_update/run records a HIP
+ * event pair around it on the context stream. */
+fc_status fc_context_enable_timing(fc_context* ctx, int enable);
+fc_status fc_context_reset_timing(fc_context* ctx);
+/* Synchronises, then fills up to max entries; *count receives the number of
+ * stages with data. */
+fc_status fc_context_get_timing(fc_context* ctx, fc_stage_timing* out, int max, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FC_DEM_H */
