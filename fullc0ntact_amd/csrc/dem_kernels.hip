@@ -7,6 +7,9 @@
 // tables are uint32 arrays.  Parameters arrive by value as kernel arguments.
 #include "fc_internal.hpp"
 #include "fc_device.hpp"
+#include "radix_common.hpp"
+
+#include <cmath>
 
 namespace fc {
 
@@ -61,31 +64,50 @@ __global__ __launch_bounds__(kBlock) void k_calc_hash(fc_sim_params P, uint32_t*
 }
 
 // integrate + calcHash in one pass (the update order runs them back to back
-// on the same arrays, so fusing them changes no value).  When
-// prevSortedHash is given, the cells occupied after the previous update are
-// reset to EMPTY: cell h was written iff h starts a run in the sorted hash
-// array, so exactly those entries are cleared (instead of the reference's
-// memset of the whole table, particleSystem.cu:198).
+// on the same arrays, so fusing them changes no value), with two more jobs
+// folded into the same read of the particles:
+//  * when prevSortedHash is given, the cells occupied after the previous
+//    update are reset to EMPTY: cell h was written iff h starts a run in the
+//    sorted hash array, so exactly those entries are cleared (instead of the
+//    reference's memset of the whole table, particleSystem.cu:198);
+//  * the digit histogram of the new keys for the radix sort's passes.
+// Each thread handles kIhPer particles strided by the block size.
+constexpr int kIhPer = 4;
+
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     fc_sim_params P, float4* __restrict__ pos, float4* __restrict__ vel, float dt,
     uint32_t* __restrict__ keysOut, const uint32_t* __restrict__ prevSortedHash,
-    uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t n) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  if (prevSortedHash != nullptr) {
-    uint32_t h = prevSortedHash[i];
-    if (i == 0 || prevSortedHash[i - 1] != h) {
-      cellStart[h] = CELL_EMPTY;
-      cellEnd[h] = CELL_EMPTY;
+    uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t n, int passes,
+    uint32_t* gHist, uint32_t* offsets, uint32_t* counters, uint32_t* devFlags) {
+  __shared__ uint32_t s_hist[RS_MAX_PASSES * RS_RADIX];
+  __shared__ uint32_t s_scratch[4];
+  for (int i = threadIdx.x; i < RS_MAX_PASSES * RS_RADIX; i += kBlock) s_hist[i] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kIhPer; ++u) {
+    const uint32_t i = blockIdx.x * (kBlock * kIhPer) + u * kBlock + threadIdx.x;
+    uint32_t key = 0;
+    if (i < n) {
+    if (prevSortedHash != nullptr) {
+      const uint32_t h = prevSortedHash[i];
+      if (i == 0 || prevSortedHash[i - 1] != h) {
+        cellStart[h] = CELL_EMPTY;
+        cellEnd[h] = CELL_EMPTY;
+      }
     }
+    const float4 p4 = pos[i];
+    const float4 v4 = vel[i];
+    v3f p = xyz(p4), v = xyz(v4);
+    integrate_box(P, dt, p, v);
+    pos[i] = make_float4(p.x, p.y, p.z, p4.w);
+    vel[i] = make_float4(v.x, v.y, v.z, v4.w);
+    key = hash_of(P, p.x, p.y, p.z);
+    keysOut[i] = key;
+    }
+    histogram_add_key(s_hist, passes, key, i < n);
   }
-  float4 p4 = pos[i];
-  float4 v4 = vel[i];
-  v3f p = xyz(p4), v = xyz(v4);
-  integrate_box(P, dt, p, v);
-  pos[i] = make_float4(p.x, p.y, p.z, p4.w);
-  vel[i] = make_float4(v.x, v.y, v.z, v4.w);
-  keysOut[i] = hash_of(P, p.x, p.y, p.z);
+  __syncthreads();
+  histogram_flush_and_finish(s_hist, passes, n, gHist, offsets, counters, devFlags, s_scratch);
 }
 
 // reorderDataAndFindCellStartD (particles_kernel.cu:119-182).  The
@@ -114,16 +136,45 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
   sortedVel[k] = oldVel[src];
 }
 
+// Sum of the pair forces of cells [s, e) on particle k, in index order,
+// with a one-ahead prefetch of the next neighbour's position and velocity.
+__device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel, uint32_t k,
+                                        uint32_t s, uint32_t e, const float4* __restrict__ sp,
+                                        const float4* __restrict__ sv, float r) {
+  v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+  if (s >= e) return cf;
+  float4 pn = sp[s], vn = sv[s];
+  for (uint32_t j = s; j < e; ++j) {
+    const float4 pc = pn, vc = vn;
+    if (j + 1 < e) {
+      pn = sp[j + 1];
+      vn = sv[j + 1];
+    }
+    if (j != k) {
+      v3f f = collide_spheres(pos, xyz(pc), vel, xyz(vc), r, r, P.spring, P.damping, P.shear,
+                              P.attraction);
+      cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+    }
+  }
+  return cf;
+}
+
 // collideD / collideCell (particles_kernel.cu:223-293): one lane per sorted
 // particle; for z, y, x in {-1,0,1} (that nesting is the summation order),
 // each neighbour cell's forces are summed into a per-cell partial that is
 // then added to the total, then the collider sphere is added, and
-// vel + force is scattered to the particle's original slot.
-__global__ __launch_bounds__(kBlock) void k_collide(
+// vel + force is scattered to the particle's original slot.  The 9
+// cell-table entries of one z-plane of
+// the neighbourhood (start and end) are loaded together before any
+// neighbour is visited, so each plane costs one table latency instead of a
+// chain of 9-18 dependent loads.  Empty cells contribute nothing: their
+// partial sum is +0 and the running total is never -0, so skipping the add
+// leaves every bit unchanged.
+__global__ __launch_bounds__(kBlock) void k_collide_v2(
     fc_sim_params P, float4* __restrict__ newVel, const float4* __restrict__ sortedPos,
     const float4* __restrict__ sortedVel, const uint32_t* __restrict__ index,
     const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n) {
-  uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
   const v3f pos = xyz(sortedPos[k]);
   const v3f vel = xyz(sortedVel[k]);
@@ -132,23 +183,20 @@ __global__ __launch_bounds__(kBlock) void k_collide(
   const int gz = cell_coord(pos.z, P.worldOrigin.z, P.cellSize.z);
   const float r = P.particleRadius;
   v3f force = make_v3(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
   for (int dz = -1; dz <= 1; ++dz) {
-    for (int dy = -1; dy <= 1; ++dy) {
-      for (int dx = -1; dx <= 1; ++dx) {
-        uint32_t h = grid_hash(gx + dx, gy + dy, gz + dz, P.gridSize);
-        uint32_t start = cellStart[h];
-        v3f cf = make_v3(0.0f, 0.0f, 0.0f);
-        if (start != CELL_EMPTY) {
-          uint32_t end = cellEnd[h];
-          if (end > n) end = n;  // consistent tables never exceed n
-          for (uint32_t j = start; j < end; ++j) {
-            if (j != k) {
-              v3f f = collide_spheres(pos, xyz(sortedPos[j]), vel, xyz(sortedVel[j]), r, r,
-                                      P.spring, P.damping, P.shear, P.attraction);
-              cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
-            }
-          }
-        }
+    uint32_t st[9], en[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const uint32_t h = grid_hash(gx + (c % 3) - 1, gy + (c / 3) - 1, gz + dz, P.gridSize);
+      st[c] = cellStart[h];
+      en[c] = cellEnd[h];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      if (st[c] != CELL_EMPTY) {
+        const uint32_t e = en[c] > n ? n : en[c];
+        v3f cf = cell_sum(P, pos, vel, k, st[c], e, sortedPos, sortedVel, r);
         force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
       }
     }
@@ -157,8 +205,152 @@ __global__ __launch_bounds__(kBlock) void k_collide(
   v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), r, P.colliderRadius,
                           P.spring, P.damping, P.shear, 0.0f);
   force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
-  uint32_t orig = index[k];
+  const uint32_t orig = index[k];
   newVel[orig] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
+}
+
+// Capacity of the per-lane contact list of k_collide_v3.
+constexpr int kListCap = 32;
+// Candidates whose loads are issued together in k_collide_v3.
+constexpr int kBatch = 4;
+
+// One lane per sorted particle, two phases.
+//  A: for each of the 9 neighbour rows, the candidates of its 3 cells are
+//     walked as one flattened sequence; a candidate whose squared distance
+//     is >= contactD2 (the smallest float whose square root reaches the
+//     contact distance, so sqrtf(d2) < R implies d2 < contactD2) cannot
+//     touch and is dropped.  Survivors are appended to a per-lane list in
+//     LDS with their cell number (0..26).
+//  B: the list is walked flat: each entry gets the reference pair force and
+//     the per-cell partial is added to the total whenever the cell number
+//     changes.  Dropped candidates contributed +0 to a partial that is never
+//     -0, and an all-dropped cell's +0 partial to a total that is never -0,
+//     so the result is bit-identical to the 27-cell loop, while the wave
+//     spends the expensive pair arithmetic only on (near-)contacts.
+// A lane whose list would overflow falls back to the direct 27-cell loop.
+__global__ __launch_bounds__(kBlock) void k_collide_v3(
+    fc_sim_params P, float contactD2, float4* __restrict__ newVel,
+    const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
+    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
+    const uint32_t* __restrict__ cellEnd, uint32_t n) {
+  __shared__ uint32_t s_list[kListCap][kBlock];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const v3f pos = xyz(sortedPos[k]);
+  const v3f vel = xyz(sortedVel[k]);
+  const int gx = cell_coord(pos.x, P.worldOrigin.x, P.cellSize.x);
+  const int gy = cell_coord(pos.y, P.worldOrigin.y, P.cellSize.y);
+  const int gz = cell_coord(pos.z, P.worldOrigin.z, P.cellSize.z);
+  const float r = P.particleRadius;
+
+  // ---- phase A ----
+  // All 27 table entries are requested before any is used (one latency).
+  uint32_t st[27], en[27];
+#pragma unroll
+  for (int c = 0; c < 27; ++c) {
+    const uint32_t h =
+        grid_hash(gx + (c % 3) - 1, gy + (c / 3) % 3 - 1, gz + c / 9 - 1, P.gridSize);
+    st[c] = cellStart[h];
+    en[c] = cellEnd[h];
+  }
+  uint32_t cnt = 0;
+  bool overflow = false;
+#pragma unroll
+  for (int row = 0; row < 9; ++row) {
+    const uint32_t s0 = st[row * 3], s1 = st[row * 3 + 1], s2 = st[row * 3 + 2];
+    const uint32_t e0 = en[row * 3], e1 = en[row * 3 + 1], e2 = en[row * 3 + 2];
+    const uint32_t l0 = s0 == CELL_EMPTY ? 0u : (e0 > n ? n : e0) - s0;
+    const uint32_t l1 = s1 == CELL_EMPTY ? 0u : (e1 > n ? n : e1) - s1;
+    const uint32_t l2 = s2 == CELL_EMPTY ? 0u : (e2 > n ? n : e2) - s2;
+    const uint32_t total = l0 + l1 + l2;
+    const uint32_t cellBase = (uint32_t)(row * 3);
+    for (uint32_t t0 = 0; t0 < total; t0 += kBatch) {
+      float4 q[kBatch];
+      uint32_t jj[kBatch], cc[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {  // kBatch independent loads in flight
+        const uint32_t t = t0 + u;
+        uint32_t j, cell;
+        if (t < l0) { j = s0 + t; cell = cellBase; }
+        else if (t < l0 + l1) { j = s1 + (t - l0); cell = cellBase + 1; }
+        else { j = s2 + (t - l0 - l1); cell = cellBase + 2; }
+        jj[u] = j;
+        cc[u] = cell;
+        if (t < total) q[u] = sortedPos[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (t0 + u < total) {
+          const float dx = q[u].x - pos.x, dy = q[u].y - pos.y, dzz = q[u].z - pos.z;
+          float d2 = dx * dx;
+          d2 = d2 + dy * dy;
+          d2 = d2 + dzz * dzz;
+          if (d2 < contactD2 && jj[u] != k) {
+            if (cnt < (uint32_t)kListCap) s_list[cnt][threadIdx.x] = jj[u] | (cc[u] << 27);
+            else overflow = true;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+
+  v3f force = make_v3(0.0f, 0.0f, 0.0f);
+  if (!overflow) {
+    // ---- phase B ----
+    v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+    uint32_t curCell = 0xffffffffu;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += kBatch) {
+      float4 qp[kBatch], qv[kBatch];
+      uint32_t ww[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (e0 + u < cnt) {
+          ww[u] = s_list[e0 + u][threadIdx.x];
+          const uint32_t j = ww[u] & 0x07ffffffu;
+          qp[u] = sortedPos[j];
+          qv[u] = sortedVel[j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (e0 + u < cnt) {
+          const uint32_t cell = ww[u] >> 27;
+          if (cell != curCell) {
+            force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+            cf = make_v3(0.0f, 0.0f, 0.0f);
+            curCell = cell;
+          }
+          v3f f = collide_spheres(pos, xyz(qp[u]), vel, xyz(qv[u]), r, r, P.spring, P.damping,
+                                  P.shear, P.attraction);
+          cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+        }
+      }
+    }
+    force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+  } else {
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const uint32_t h = grid_hash(gx + dx, gy + dy, gz + dz, P.gridSize);
+          const uint32_t start = cellStart[h];
+          if (start == CELL_EMPTY) continue;
+          const uint32_t end = cellEnd[h] > n ? n : cellEnd[h];
+          v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+          for (uint32_t j = start; j < end; ++j)
+            if (j != k) {
+              v3f f = collide_spheres(pos, xyz(sortedPos[j]), vel, xyz(sortedVel[j]), r, r,
+                                      P.spring, P.damping, P.shear, P.attraction);
+              cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+            }
+          force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+        }
+  }
+  v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
+  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), r, P.colliderRadius,
+                          P.spring, P.damping, P.shear, 0.0f);
+  force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
+  newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
 }  // namespace
@@ -190,10 +382,12 @@ hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* has
 
 hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
                                  float dt, uint32_t* keysOut, const uint32_t* prevSortedHash,
-                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n) {
+                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
+                                 SortScratch& sort, int passes, uint32_t* devFlags) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_integrate_hash, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, pos, vel,
-                     dt, keysOut, prevSortedHash, cellStart, cellEnd, n);
+  hipLaunchKernelGGL(k_integrate_hash, dim3(grid_for(n, kBlock * kIhPer)), dim3(kBlock), 0, s, P,
+                     pos, vel, dt, keysOut, prevSortedHash, cellStart, cellEnd, n, passes,
+                     sort.histogram, sort.offsets, sort.counters, devFlags);
   return hipGetLastError();
 }
 
@@ -207,13 +401,33 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
   return hipGetLastError();
 }
 
+// Smallest float t with sqrtf(t) >= R: every d2 >= t has sqrtf(d2) >= R
+// (sqrtf is correctly rounded and monotone), so d2 < t is a lossless
+// pre-test for the contact condition sqrtf(d2) < R.
+float contact_d2_threshold(float R) {
+  if (!(R > 0.0f)) return 0.0f;
+  if (std::isinf(R)) return INFINITY;
+  float t = R * R;
+  while (t > 0.0f && std::sqrt(std::nextafter(t, 0.0f)) >= R) t = std::nextafter(t, 0.0f);
+  while (std::sqrt(t) < R) t = std::nextafter(t, INFINITY);
+  return t;
+}
+
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
                           const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_collide, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, newVel,
-                     sortedPos, sortedVel, index, cellStart, cellEnd, n);
+  dim3 g(grid_for(n, kBlock)), b(kBlock);
+  // The contact-list kernel packs a sorted index into 27 bits and assumes
+  // the three cells of a neighbour row are distinct (G.x >= 4).
+  if (n >= (1u << 27) || P.gridSize.x < 4)
+    hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
+                       cellStart, cellEnd, n);
+  else
+    hipLaunchKernelGGL(k_collide_v3, g, b, 0, s, P,
+                       contact_d2_threshold(P.particleRadius + P.particleRadius), newVel,
+                       sortedPos, sortedVel, index, cellStart, cellEnd, n);
   return hipGetLastError();
 }
 

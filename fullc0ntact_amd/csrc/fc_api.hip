@@ -188,6 +188,8 @@ fc_status check_device_flags(fc_context* ctx) {
   FC_HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (flags & DEVERR_SORT_STALLED)
     return set_error(FC_ERROR_SORT_STALLED, "radix sort look-back exceeded its spin bound");
+  if (flags & DEVERR_HIST_STALLED)
+    return set_error(FC_ERROR_SORT_STALLED, "radix sort histogram never completed");
   return set_error(FC_ERROR_INVALID_VALUE,
                    "a sort key had bits above log2(numCells): hashes must be < numCells");
 }
@@ -230,8 +232,10 @@ fc_status fc_context_create(int device, fc_context** out) {
   SortScratch& s = ctx->sort;
   bool ok = hipMalloc(&ctx->devFlags, sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&s.counters, RSC_COUNT * sizeof(uint32_t)) == hipSuccess &&
-            hipMalloc(&s.partial, (size_t)RS_MAX_HIST_BLOCKS * RS_MAX_PASSES * RS_RADIX *
-                                      sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&s.histogram, (size_t)RS_HIST_COPIES * RS_MAX_PASSES * RS_RADIX *
+                                        sizeof(uint32_t)) == hipSuccess &&
+            hipMemset(s.histogram, 0, (size_t)RS_HIST_COPIES * RS_MAX_PASSES * RS_RADIX *
+                                          sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&s.offsets, (size_t)RS_MAX_PASSES * RS_RADIX * sizeof(uint32_t)) ==
                 hipSuccess &&
             hipMemset(ctx->devFlags, 0, sizeof(uint32_t)) == hipSuccess &&
@@ -254,7 +258,7 @@ fc_status fc_context_destroy(fc_context* ctx) {
   hipFree(s.keysB);
   hipFree(s.valsB);
   hipFree(s.status);
-  hipFree(s.partial);
+  hipFree(s.histogram);
   hipFree(s.offsets);
   hipFree(s.counters);
   hipFree(ctx->devFlags);

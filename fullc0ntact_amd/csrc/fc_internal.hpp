@@ -37,6 +37,7 @@ fc_status hip_error(hipError_t e, const char* what);
 enum : uint32_t {
   DEVERR_SORT_STALLED = 1u << 0,   // look-back exceeded its spin bound
   DEVERR_SORT_KEY_RANGE = 1u << 1, // a key had bits above the sorted width
+  DEVERR_HIST_STALLED = 1u << 2,   // histogram totals never reached n
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
@@ -44,13 +45,14 @@ constexpr uint32_t CELL_EMPTY = 0xffffffffu;
 // ---------------------------------------------------------------------------
 // Radix sort (radix_sort.hip).
 // ---------------------------------------------------------------------------
-constexpr int RS_THREADS = 256;               // 4 wave64s
-constexpr int RS_ITEMS = 16;                  // keys per thread per tile
+constexpr int RS_THREADS = 1024;              // 16 wave64s
+constexpr int RS_ITEMS = 4;                   // keys per thread per tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS; // 4096 keys
 constexpr int RS_RADIX_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_RADIX_BITS;
 constexpr int RS_MAX_PASSES = 4;
-constexpr int RS_MAX_HIST_BLOCKS = 64;
+constexpr int RS_MAX_HIST_BLOCKS = 512;
+constexpr int RS_HIST_COPIES = 32;  // privatised copies of the global histogram
 
 // Device counters used by the sort; all self-resetting (see radix_sort.hip).
 enum : int {
@@ -68,7 +70,7 @@ struct SortScratch {
   size_t capacity = 0;               // elements in keysA/valsA/keysB/valsB
   uint64_t* status = nullptr;        // tile x digit look-back words
   size_t statusCapacity = 0;         // in words
-  uint32_t* partial = nullptr;       // [RS_MAX_HIST_BLOCKS][RS_MAX_PASSES][RS_RADIX]
+  uint32_t* histogram = nullptr;     // [RS_HIST_COPIES][RS_MAX_PASSES][RS_RADIX], zero between sorts
   uint32_t* offsets = nullptr;       // [RS_MAX_PASSES][RS_RADIX] exclusive bucket starts
   uint32_t* counters = nullptr;      // [RSC_COUNT]
 };
@@ -78,15 +80,20 @@ struct StageTimer;
 // Number of significant bits needed for keys in [0, numCells).
 int key_bits_for_cells(uint32_t numCells);
 
+// Number of scatter passes for `bits`-bit keys.
+int radix_passes(int bits, bool inPlace);
+
 // Stable ascending LSD sort of (key, value) pairs on the low `bits` bits.
 // valsIn == nullptr means the values are the identity 0..n-1.  keysIn may
 // equal keysOut and valsIn may equal valsOut (in-place); intermediate passes
 // use the scratch buffers.  All launches go to `stream`, nothing is
 // allocated, nothing synchronises: the call is graph-capturable.
+// histogramReady: the digit histogram of keysIn was already produced (by
+// the kernel that wrote the keys), so the histogram launch is skipped.
 hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
                             const uint32_t* keysIn, const uint32_t* valsIn,
                             uint32_t* keysOut, uint32_t* valsOut, uint32_t n, int bits,
-                            StageTimer* timer);
+                            StageTimer* timer, bool histogramReady = false);
 
 // ---------------------------------------------------------------------------
 // Per-stage timing with HIP events.
@@ -122,10 +129,13 @@ hipError_t launch_integrate(hipStream_t s, const fc_sim_params& P, float4* pos, 
 hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* hash,
                             uint32_t* index, const float4* pos, uint32_t n);
 // integrate + calcHash fused, plus clearing of the cells listed in the
-// previous step's sorted hashes (prevSortedHash may be null: no clearing).
+// previous step's sorted hashes (prevSortedHash may be null: no clearing),
+// plus the radix sort's digit histogram of the new keys for `passes`
+// passes (written into the sort scratch, as k_rs_histogram would).
 hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
                                  float dt, uint32_t* keysOut, const uint32_t* prevSortedHash,
-                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n);
+                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
+                                 SortScratch& sort, int passes, uint32_t* devFlags);
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
@@ -135,6 +145,7 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n);
 
+float contact_d2_threshold(float R);
 fc_status validate_params(const fc_sim_params& P);
 // Grows the context's sort scratch to hold n keys (synchronises first).
 fc_status ensure_sort_capacity(fc_context* ctx, size_t n);

@@ -88,13 +88,14 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
     sys->tablesTrackHash = false;
     return FC_SUCCESS;
   }
+  const int bits = key_bits_for_cells(P.numCells);
   if (timer) timer->begin(st, "integrate_hash");
   e = launch_integrate_hash(st, P, sys->pos, sys->vel, dt, sys->keys, prev, sys->cellStart,
-                            sys->cellEnd, n);
+                            sys->cellEnd, n, ctx->sort, radix_passes(bits, false), ctx->devFlags);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "integrate_hash");
   e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index, n,
-                       key_bits_for_cells(P.numCells), timer);
+                       bits, timer, /*histogramReady=*/true);
   if (e != hipSuccess) return hip_error(e, "sort");
   if (timer) timer->begin(st, "reorder");
   e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,

@@ -5,11 +5,11 @@
 // lexicographic (hash, original index) order.
 //
 // Structure (one histogram launch + one launch per 8-bit digit):
-//   k_rs_histogram  every block counts all digits of its share of the keys
-//                   into LDS with wave-aggregated adds, writes a partial
+//   k_rs_histogram  every block counts all digits of its tiles into LDS with
+//                   wave-aggregated adds and adds them to a global
 //                   histogram; the last block to finish (ticket counter)
-//                   sums the partials, writes exclusive bucket offsets per
-//                   pass and bumps the sort epoch.
+//                   turns it into exclusive bucket offsets per pass, zeroes
+//                   it for the next sort and bumps the sort epoch.
 //   k_rs_scatter    one 4096-key tile per block, tiles taken in launch order
 //                   from a ticket counter.  Each wave ranks its 1024 keys
 //                   with ballot-based digit matching into a wave-private LDS
@@ -28,6 +28,7 @@
 // were handed to blocks that had already started.  The wait is additionally
 // bounded; hitting the bound raises DEVERR_SORT_STALLED instead of hanging.
 #include "fc_internal.hpp"
+#include "radix_common.hpp"
 
 namespace fc {
 
@@ -40,12 +41,11 @@ constexpr uint64_t kFlagPrefix = 2ull << 62;
 constexpr uint64_t kFlagMask = 3ull << 62;
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLookback = 4;
 
 __device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t value) {
   return flag | ((uint64_t)(epoch & kEpochMask) << 32) | (uint64_t)value;
 }
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 // Lanes (among `active`) whose 8-bit digit equals this lane's digit.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
@@ -59,86 +59,38 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
   return peers;
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads, 4 waves).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_waveTotals) {
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  uint32_t inc = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t t = __shfl_up(inc, off);
-    if (lane >= (uint32_t)off) inc += t;
-  }
-  if (lane == 63) s_waveTotals[wave] = inc;
-  __syncthreads();
-  uint32_t base = 0;
-  for (uint32_t w = 0; w < wave; ++w) base += s_waveTotals[w];
-  __syncthreads();
-  return base + inc - v;
-}
-
+// Standalone digit histogram (used when the keys come from the caller).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_histogram(
     const uint32_t* __restrict__ keys, uint32_t n, int passes, int bits,
-    uint32_t* __restrict__ partial, uint32_t* __restrict__ offsets, uint32_t* counters,
+    uint32_t* __restrict__ gHist, uint32_t* __restrict__ offsets, uint32_t* counters,
     uint32_t* devFlags) {
-  __shared__ uint32_t s_hist[RS_MAX_PASSES][RS_RADIX];
-  __shared__ uint32_t s_waveTotals[kWaves];
-  __shared__ uint32_t s_last;
-  for (int i = threadIdx.x; i < RS_MAX_PASSES * RS_RADIX; i += RS_THREADS)
-    (&s_hist[0][0])[i] = 0u;
+  __shared__ uint32_t s_hist[RS_MAX_PASSES * RS_RADIX];
+  __shared__ uint32_t s_scratch[RS_RADIX / 64];
+  for (int i = threadIdx.x; i < RS_MAX_PASSES * RS_RADIX; i += RS_THREADS) s_hist[i] = 0u;
   __syncthreads();
-
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t gwave = blockIdx.x * kWaves + wave;
-  const uint32_t nwaves = gridDim.x * kWaves;
   const uint32_t highMask = bits >= 32 ? 0u : ~((1u << bits) - 1u);
   uint32_t rangeErr = 0u;
-  for (uint32_t slice = gwave; (size_t)slice * 64u < n; slice += nwaves) {
-    const uint32_t idx = slice * 64u + lane;
-    const bool valid = idx < n;
-    const uint32_t key = valid ? keys[idx] : 0u;
-    rangeErr |= key & highMask;
-    const uint64_t active = __ballot(valid);
-    for (int p = 0; p < passes; ++p) {
-      const uint32_t d = (key >> (p * RS_RADIX_BITS)) & (RS_RADIX - 1);
-      const uint64_t peers = match_digit(d, active);
-      const uint64_t lower = peers & ((1ull << lane) - 1ull);
-      if (valid && lower == 0ull) atomicAdd(&s_hist[p][d], (uint32_t)__popcll(peers));
+  const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
+  for (uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x) {
+    uint32_t key[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = tile * RS_TILE + i * RS_THREADS + threadIdx.x;
+      key[i] = idx < n ? keys[idx] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = tile * RS_TILE + i * RS_THREADS + threadIdx.x;
+      if (idx < n) rangeErr |= key[i] & highMask;
+      histogram_add_key(s_hist, passes, key[i], idx < n);
     }
   }
   if (rangeErr) atomicOr(devFlags, DEVERR_SORT_KEY_RANGE);
   __syncthreads();
-
-  uint32_t* mine = partial + (size_t)blockIdx.x * RS_MAX_PASSES * RS_RADIX;
-  for (int i = threadIdx.x; i < passes * RS_RADIX; i += RS_THREADS) mine[i] = (&s_hist[0][0])[i];
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t ticket = atomicAdd(&counters[RSC_HIST_DONE], 1u);
-    s_last = (ticket == gridDim.x - 1u) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;
-
-  // Last block: every other block's partial is visible after its fence and
-  // ticket; sum them and publish exclusive bucket offsets per pass.
-  __threadfence();
-  const uint32_t d = threadIdx.x;
-  for (int p = 0; p < passes; ++p) {
-    uint32_t total = 0;
-    for (uint32_t b = 0; b < gridDim.x; ++b)
-      total += __hip_atomic_load(&partial[((size_t)b * RS_MAX_PASSES + p) * RS_RADIX + d],
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t excl = block_exclusive_scan(total, s_waveTotals);
-    offsets[p * RS_RADIX + d] = excl;
-  }
-  if (threadIdx.x == 0) {
-    counters[RSC_HIST_DONE] = 0u;
-    counters[RSC_EPOCH] = counters[RSC_EPOCH] + 1u;
-  }
+  histogram_flush_and_finish(s_hist, passes, n, gHist, offsets, counters, devFlags, s_scratch);
 }
 
+// One LSD pass over one 4096-key tile per block (16 waves x 4 keys each).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int pass,
@@ -149,11 +101,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
   __shared__ uint32_t s_tileStart[RS_RADIX];
   __shared__ uint32_t s_digitBase[RS_RADIX];
-  __shared__ uint32_t s_waveTotals[kWaves];
+  __shared__ uint32_t s_scratch[RS_RADIX / 64];
   __shared__ uint32_t s_tile;
 
   const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
-  const uint32_t lane = lane_id();
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const int shift = pass * RS_RADIX_BITS;
 
@@ -192,53 +144,67 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   }
   __syncthreads();
 
-  // Per digit: wave offsets inside the tile and the tile total.
+  // Threads 0..255 own one digit each: wave offsets inside the tile, the
+  // tile total, publication and look-back.
   const uint32_t d = threadIdx.x;
-  uint32_t running = 0;
+  const bool digitThread = d < (uint32_t)RS_RADIX;
+  uint32_t tileCount = 0;
+  uint64_t* myStatus = status + (size_t)tile * RS_RADIX + (digitThread ? d : 0u);
+  if (digitThread) {
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    uint32_t c = s_waveHist[w][d];
-    s_waveHist[w][d] = running;
-    running += c;
-  }
-  const uint32_t tileCount = running;
-  uint64_t* myStatus = status + (size_t)tile * RS_RADIX + d;
-  if (tile == 0u) {
-    __hip_atomic_store(myStatus, pack_status(kFlagPrefix, epoch, tileCount), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    __hip_atomic_store(myStatus, pack_status(kFlagAgg, epoch, tileCount), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  const uint32_t tileStart = block_exclusive_scan(tileCount, s_waveTotals);
-  s_tileStart[d] = tileStart;
-
-  // Decoupled look-back for this digit.
-  uint32_t prefix = 0;
-  if (tile > 0u) {
-    int t = (int)tile - 1;
-    uint32_t spins = 0;
-    while (t >= 0) {
-      const uint64_t w = __hip_atomic_load(status + (size_t)t * RS_RADIX + d, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t flag = w & kFlagMask;
-      const uint32_t wEpoch = (uint32_t)(w >> 32) & kEpochMask;
-      if (flag == 0ull || wEpoch != epoch) {
-        if (++spins > kSpinLimit) {
-          atomicOr(devFlags, DEVERR_SORT_STALLED);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      prefix += (uint32_t)w;
-      if (flag == kFlagPrefix) break;
-      --t;
+    for (int w = 0; w < kWaves; ++w) {
+      uint32_t c = s_waveHist[w][d];
+      s_waveHist[w][d] = tileCount;
+      tileCount += c;
     }
-    __hip_atomic_store(myStatus, pack_status(kFlagPrefix, epoch, prefix + tileCount),
+    __hip_atomic_store(myStatus, pack_status(tile == 0u ? kFlagPrefix : kFlagAgg, epoch, tileCount),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  s_digitBase[d] = offsets[pass * RS_RADIX + d] + prefix - tileStart;
+  const uint32_t tileStart = scan256_exclusive(digitThread ? tileCount : 0u, s_scratch);
+  if (digitThread) {
+    s_tileStart[d] = tileStart;
+    // Decoupled look-back for this digit, reading kLookback predecessor
+    // words per round trip and consuming them newest to oldest until an
+    // inclusive prefix is found.
+    uint32_t prefix = 0;
+    if (tile > 0u) {
+      int t = (int)tile - 1;
+      uint32_t spins = 0;
+      for (;;) {
+        uint64_t w[kLookback];
+#pragma unroll
+        for (int i = 0; i < kLookback; ++i)
+          w[i] = (t - i >= 0) ? __hip_atomic_load(status + (size_t)(t - i) * RS_RADIX + d,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < kLookback; ++i) {
+          if (done || used != i) break;
+          if (t - i < 0) { done = true; break; }  // unreachable: tile 0 holds a prefix
+          const uint64_t flag = w[i] & kFlagMask;
+          const uint32_t wEpoch = (uint32_t)(w[i] >> 32) & kEpochMask;
+          if (flag == 0ull || wEpoch != epoch) break;  // not published yet
+          prefix += (uint32_t)w[i];
+          ++used;
+          if (flag == kFlagPrefix) done = true;
+        }
+        if (done) break;
+        t -= used;
+        if (used == 0) {
+          if (++spins > kSpinLimit) {
+            atomicOr(devFlags, DEVERR_SORT_STALLED);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __hip_atomic_store(myStatus, pack_status(kFlagPrefix, epoch, prefix + tileCount),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_digitBase[d] = offsets[pass * RS_RADIX + d] + prefix - tileStart;
+  }
   __syncthreads();
 
   // Stage the tile digit-sorted in LDS.
@@ -282,28 +248,36 @@ int key_bits_for_cells(uint32_t numCells) {
   return b;
 }
 
-hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
-                            const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut,
-                            uint32_t* valsOut, uint32_t n, int bits, StageTimer* timer) {
-  if (n == 0) return hipSuccess;
+int radix_passes(int bits, bool inPlace) {
   int passes = (bits + RS_RADIX_BITS - 1) / RS_RADIX_BITS;
   if (passes < 1) passes = 1;  // still produces the (identity) values
   // A single pass cannot run in place; a second pass over an all-zero digit
   // is a stable copy, so in-place one-digit sorts take two passes.
-  const bool inPlace = keysIn == keysOut || (valsIn != nullptr && valsIn == valsOut);
   if (passes == 1 && inPlace) passes = 2;
+  return passes;
+}
+
+hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
+                            const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut,
+                            uint32_t* valsOut, uint32_t n, int bits, StageTimer* timer,
+                            bool histogramReady) {
+  if (n == 0) return hipSuccess;
+  const bool inPlace = keysIn == keysOut || (valsIn != nullptr && valsIn == valsOut);
+  const int passes = radix_passes(bits, inPlace);
   if (passes > RS_MAX_PASSES) return hipErrorInvalidValue;
   const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
   if (s.capacity < n || s.statusCapacity < (size_t)numTiles * RS_RADIX) return hipErrorInvalidValue;
-
-  uint32_t histBlocks = (n + RS_TILE - 1) / RS_TILE;
-  if (histBlocks > (uint32_t)RS_MAX_HIST_BLOCKS) histBlocks = RS_MAX_HIST_BLOCKS;
-  if (timer) timer->begin(stream, "sort_histogram");
-  hipLaunchKernelGGL(k_rs_histogram, dim3(histBlocks), dim3(RS_THREADS), 0, stream, keysIn, n,
-                     passes, bits, s.partial, s.offsets, s.counters, devFlags);
-  hipError_t e = hipGetLastError();
-  if (timer) timer->end(stream);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (!histogramReady) {
+    uint32_t histBlocks = numTiles;
+    if (histBlocks > (uint32_t)RS_MAX_HIST_BLOCKS) histBlocks = RS_MAX_HIST_BLOCKS;
+    if (timer) timer->begin(stream, "sort_histogram");
+    hipLaunchKernelGGL(k_rs_histogram, dim3(histBlocks), dim3(RS_THREADS), 0, stream, keysIn, n,
+                       passes, bits, s.histogram, s.offsets, s.counters, devFlags);
+    e = hipGetLastError();
+    if (timer) timer->end(stream);
+    if (e != hipSuccess) return e;
+  }
 
   // Ping-pong so that the last pass lands in keysOut/valsOut and no pass
   // reads and writes the same buffer.

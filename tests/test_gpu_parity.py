@@ -331,3 +331,73 @@ def test_system_param_validation(gpu_ctx):
     with pytest.raises(FcError):
         ps.setArray(ps.POSITION, np.zeros((4, 4), np.float32), 8, 4)  # past the end
     ps.close()
+
+
+def test_collide_contact_distance_edges(gpu_ctx):
+    # Pairs placed at separations straddling R = 2r to the last ulp: the
+    # collide's distance pre-test must drop no pair the reference counts.
+    n_pairs, grid, r = 400, 64, 1.0 / 64.0
+    P = sim_params(2 * n_pairs, grid, r, colliderPos=(50.0, 50.0, 50.0))
+    OP = orc.params_from(P)
+    S = Stages(gpu_ctx, P)
+    R = np.float32(2 * r)
+    rng = np.random.default_rng(21)
+    pos = np.zeros((2 * n_pairs, 4), np.float32)
+    vel = np.zeros((2 * n_pairs, 4), np.float32)
+    for p in range(n_pairs):
+        c = np.array([-0.9 + 0.09 * (p % 20), -0.9 + 0.09 * ((p // 20) % 20), 0.0], np.float32)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        d = R
+        for _ in range(int(rng.integers(-3, 4))):
+            d = np.nextafter(d, np.float32(np.inf) if rng.random() < 0.5 else np.float32(0))
+        pos[2 * p, :3] = c
+        pos[2 * p + 1, :3] = (c + (u * d).astype(np.float32)).astype(np.float32)
+        vel[2 * p:2 * p + 2, :3] = rng.uniform(-0.01, 0.01, size=(2, 3))
+    h, i = orc.calc_hash(pos, OP)
+    h, i = orc.sort_pairs(h, i)
+    cs, ce, sp, sv = orc.reorder(h, i, pos, vel, P.numCells)
+    nv = S.collide(sp, sv, i, cs, ce, vel)
+    onv = orc.collide(sp, sv, i, cs, ce, OP, newVel4=vel)
+    assert np.array_equal(bits(nv), bits(onv))
+    assert np.any(nv[:, :3] != vel[:, :3])  # some pairs touch
+
+
+def test_collide_list_overflow_path(gpu_ctx):
+    # 40 overlapping spheres in one cell: every particle has more near
+    # neighbours than the per-lane contact list holds.
+    n, grid, r = 40, 16, 1.0 / 16.0
+    P = sim_params(n, grid, r)
+    OP = orc.params_from(P)
+    S = Stages(gpu_ctx, P)
+    rng = np.random.default_rng(5)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = 0.02 + rng.uniform(0, 0.05, size=(n, 3))
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.uniform(-0.01, 0.01, size=(n, 3))
+    h, i = orc.calc_hash(pos, OP)
+    h, i = orc.sort_pairs(h, i)
+    cs, ce, sp, sv = orc.reorder(h, i, pos, vel, P.numCells)
+    nv = S.collide(sp, sv, i, cs, ce, vel)
+    onv = orc.collide(sp, sv, i, cs, ce, OP, newVel4=vel)
+    assert np.array_equal(bits(nv), bits(onv))
+
+
+def test_collide_tiny_grid_visits_wrapped_cells_twice(gpu_ctx):
+    # G = 2: cells x-1 and x+1 are the same cell, so the reference loop adds
+    # that cell's forces twice; the fallback kernel must do exactly that.
+    n = 300
+    P = sim_params(n, 2, 0.06, gridSize=(2, 2, 2), numCells=8, cellSize=(1.0, 1.0, 1.0))
+    OP = orc.params_from(P)
+    S = Stages(gpu_ctx, P)
+    rng = np.random.default_rng(8)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(-0.9, 0.9, size=(n, 3))
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.uniform(-0.01, 0.01, size=(n, 3))
+    h, i = orc.calc_hash(pos, OP)
+    h, i = orc.sort_pairs(h, i)
+    cs, ce, sp, sv = orc.reorder(h, i, pos, vel, P.numCells)
+    nv = S.collide(sp, sv, i, cs, ce, vel)
+    onv = orc.collide(sp, sv, i, cs, ce, OP, newVel4=vel)
+    assert np.array_equal(bits(nv), bits(onv))
