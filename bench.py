@@ -71,6 +71,10 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
     from fullc0ntact_amd import ParticleSystem, make_params
 
     ps = ParticleSystem(N, (GRID, GRID, GRID))
+    probe_bytes = 1 << 30  # 1 GiB per buffer: far beyond the 256 MB last-level cache
+    probe_ms = ps.ctx.copy_probe(probe_bytes, 10)
+    copy_gbs = 2 * probe_bytes / (probe_ms * 1e-3) / 1e9
+    log(f"[bench] copy probe: {copy_gbs:.0f} GB/s (read + write of 1 GiB)")
     ps.m_params = make_params(N, GRID, RADIUS)
     ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
     ps.sync()
@@ -93,12 +97,19 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
     for _ in range(timing_steps):
         ps.update(DT)
     stages = ctx.timing()
+    # The multi-step run fuses each collide with the next update's integrate;
+    # time its kernels too (logged only).
+    ctx.reset_timing()
+    ps.run(DT, timing_steps)
+    run_stages = ctx.timing()
     ctx.enable_timing(False)
     ps.sync()
+    for k, v in run_stages.items():
+        log(f"[bench] run stage {k:18s} {v[1] / max(v[0], 1) * 1e3:10.2f} us x {v[0]}")
     cs = ps.download("cellStart")
     c_occ, c_touch = cell_counts(cs, GRID)
     ps.close()
-    return dict(elapsed=elapsed, stages=stages, c_occ=c_occ, c_touch=c_touch)
+    return dict(elapsed=elapsed, stages=stages, c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs)
 
 
 def cpu_leg(target_seconds: float, threads: int) -> dict:
@@ -111,7 +122,7 @@ def cpu_leg(target_seconds: float, threads: int) -> dict:
     t0 = time.perf_counter()
     st.update(DT, P)  # warm-up step (first touch of every array)
     first = time.perf_counter() - t0
-    k = int(max(1, min(20, target_seconds / max(first, 1e-3))))
+    k = int(max(1, min(500, target_seconds / max(first, 1e-3))))
     t0 = time.perf_counter()
     st.run(DT, k, P)
     el = time.perf_counter() - t0
@@ -126,7 +137,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported "
@@ -149,7 +160,7 @@ def main(argv=None) -> int:
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "collide", "bytes_per_launch": collide_bytes,
-                "avg_launch_us": collide_us}
+                "avg_launch_us": collide_us, "copy_probe_GBs": round(g["copy_gbs"], 1)}
 
     cpu = None
     if not a.no_cpu:

@@ -128,6 +128,7 @@ _SIGS = {
     "fc_context_enable_timing": (_st, [_vp, C.c_int]),
     "fc_context_reset_timing": (_st, [_vp]),
     "fc_context_get_timing": (_st, [_vp, C.POINTER(StageTiming), C.c_int, C.POINTER(C.c_int)]),
+    "fc_copy_probe": (_st, [_vp, _vp, _vp, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
 }
 
 
@@ -196,6 +197,18 @@ class Context:
         p = SimParams()
         check(lib.fc_get_parameters(self.handle, C.byref(p)))
         return p
+
+    def copy_probe(self, nbytes: int, iterations: int = 20) -> float:
+        """Average time (ms) of a plain device-to-device copy of nbytes."""
+        src = DeviceArray(self, nbytes)
+        dst = DeviceArray(self, nbytes)
+        ms = C.c_float(0.0)
+        try:
+            check(lib.fc_copy_probe(self.handle, dst.ptr, src.ptr, nbytes, iterations, C.byref(ms)))
+        finally:
+            src.free()
+            dst.free()
+        return float(ms.value)
 
     def enable_timing(self, on: bool = True) -> None:
         check(lib.fc_context_enable_timing(self.handle, 1 if on else 0))

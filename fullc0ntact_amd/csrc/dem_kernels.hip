@@ -38,6 +38,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
   if (tid < count - tail) p[tail + tid] = value;
 }
 
+// Plain 16-byte-per-lane copy (bandwidth probe).
+__global__ __launch_bounds__(kBlock) void k_copy16(uint4* __restrict__ dst,
+                                                   const uint4* __restrict__ src, size_t count) {
+  size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) dst[i] = src[i];
+}
+
 // integrate_functor (particles_kernel.cu:39-75) over zip(pos, vel).
 __global__ __launch_bounds__(kBlock) void k_integrate(fc_sim_params P, float4* __restrict__ pos,
                                                       float4* __restrict__ vel, float dt,
@@ -78,10 +85,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     fc_sim_params P, float4* __restrict__ pos, float4* __restrict__ vel, float dt,
     uint32_t* __restrict__ keysOut, const uint32_t* __restrict__ prevSortedHash,
     uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t n, int passes,
-    uint32_t* gHist, uint32_t* offsets, uint32_t* counters, uint32_t* devFlags) {
-  __shared__ uint32_t s_hist[RS_MAX_PASSES * RS_RADIX];
-  __shared__ uint32_t s_scratch[4];
-  for (int i = threadIdx.x; i < RS_MAX_PASSES * RS_RADIX; i += kBlock) s_hist[i] = 0u;
+    uint32_t* gHist, uint32_t* counters) {
+  __shared__ uint32_t s_hist[RS_RADIX];
+  for (int i = threadIdx.x; i < RS_RADIX; i += kBlock) s_hist[i] = 0u;
+  histogram_producer_prologue(gHist, passes, counters);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < kIhPer; ++u) {
@@ -104,10 +111,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     key = hash_of(P, p.x, p.y, p.z);
     keysOut[i] = key;
     }
-    histogram_add_key(s_hist, passes, key, i < n);
+    histogram_add_digit(s_hist, key & (RS_RADIX - 1), i < n);
   }
   __syncthreads();
-  histogram_flush_and_finish(s_hist, passes, n, gHist, offsets, counters, devFlags, s_scratch);
+  histogram_flush(s_hist, gHist, 0);
 }
 
 // reorderDataAndFindCellStartD (particles_kernel.cu:119-182).  The
@@ -209,8 +216,24 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
   newVel[orig] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
+// Logical block index for XCD-aware scheduling.  The hardware hands
+// consecutive block ids to the 8 XCDs round-robin; remapping so that each
+// XCD receives runs of `group` consecutive logical blocks keeps spatially
+// adjacent sorted particles (which share neighbours) in one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t b, uint32_t nb, uint32_t group) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t span = kXcds * group;
+  const uint32_t full = nb / span * span;  // blocks in complete spans
+  if (group <= 1u || b >= full) return b;
+  const uint32_t base = b / span * span, r = b - base;
+  return base + (r % kXcds) * group + r / kXcds;
+}
+
+// Blocks per XCD run in the collide's launch order (measured best of 1-256).
+constexpr uint32_t kCollideXcdGroup = 16;
+
 // Capacity of the per-lane contact list of k_collide_v3.
-constexpr int kListCap = 32;
+constexpr int kListCap = 28;
 // Candidates whose loads are issued together in k_collide_v3.
 constexpr int kBatch = 4;
 
@@ -228,16 +251,13 @@ constexpr int kBatch = 4;
 //     so the result is bit-identical to the 27-cell loop, while the wave
 //     spends the expensive pair arithmetic only on (near-)contacts.
 // A lane whose list would overflow falls back to the direct 27-cell loop.
-__global__ __launch_bounds__(kBlock) void k_collide_v3(
-    fc_sim_params P, float contactD2, float4* __restrict__ newVel,
-    const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
-    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
-    const uint32_t* __restrict__ cellEnd, uint32_t n) {
-  __shared__ uint32_t s_list[kListCap][kBlock];
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n) return;
-  const v3f pos = xyz(sortedPos[k]);
-  const v3f vel = xyz(sortedVel[k]);
+__device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float contactD2,
+                                              uint32_t k, v3f pos, v3f vel,
+                                              const float4* __restrict__ sortedPos,
+                                              const float4* __restrict__ sortedVel,
+                                              const uint32_t* __restrict__ cellStart,
+                                              const uint32_t* __restrict__ cellEnd, uint32_t n,
+                                              uint32_t (*s_list)[kBlock]) {
   const int gx = cell_coord(pos.x, P.worldOrigin.x, P.cellSize.x);
   const int gy = cell_coord(pos.y, P.worldOrigin.y, P.cellSize.y);
   const int gz = cell_coord(pos.z, P.worldOrigin.z, P.cellSize.z);
@@ -350,7 +370,54 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
   v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), r, P.colliderRadius,
                           P.spring, P.damping, P.shear, 0.0f);
   force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
-  newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
+  return make_v3(vel.x + force.x, vel.y + force.y, vel.z + force.z);
+}
+
+// The collide stage.  newVel[index[k]] = (vel + force, 0).
+// With kFuse the kernel also performs the next update's first stage for the
+// same particle, which the update order would run next on the same arrays:
+// integrate (from the collided velocity and the unchanged position) into
+// pos/newVel, the cell hash into keysOut, and the radix sort's digit
+// histogram of those keys.  Every value written is the one the separate
+// stages would write.
+template <bool kFuse>
+__global__ __launch_bounds__(kBlock) void k_collide_v3(
+    fc_sim_params P, float contactD2, float4* __restrict__ newVel,
+    const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
+    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
+    const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
+    float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters) {
+  __shared__ uint32_t s_list[kListCap][kBlock];
+  __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
+  const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
+  const bool valid = k < n;
+  if (kFuse) {
+    for (int i = threadIdx.x; i < RS_RADIX; i += kBlock) s_hist[i] = 0u;
+    histogram_producer_prologue(gHist, passes, counters);
+  }
+  uint32_t key = 0;
+  if (valid) {
+    const float4 p4 = sortedPos[k];
+    const v3f nv = collide_one_v3(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), sortedPos,
+                                  sortedVel, cellStart, cellEnd, n, s_list);
+    const uint32_t orig = index[k];
+    if (!kFuse) {
+      newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+    } else {
+      v3f p = xyz(p4), v = nv;
+      integrate_box(P, dt, p, v);
+      pos[orig] = make_float4(p.x, p.y, p.z, p4.w);
+      newVel[orig] = make_float4(v.x, v.y, v.z, 0.0f);
+      key = hash_of(P, p.x, p.y, p.z);
+      keysOut[orig] = key;
+    }
+  }
+  if (kFuse) {
+    __syncthreads();
+    histogram_add_digit(s_hist, key & (RS_RADIX - 1), valid);
+    __syncthreads();
+    histogram_flush(s_hist, gHist, 0);
+  }
 }
 
 }  // namespace
@@ -361,6 +428,16 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t co
   unsigned grid = grid_for(vec, kBlock);
   if (grid > 2048u) grid = 2048u;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(kBlock), 0, s, p, value, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  const size_t count = bytes / 16;
+  if (count == 0) return hipSuccess;
+  unsigned grid = grid_for(count, kBlock);
+  if (grid > 8192u) grid = 8192u;
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, (uint4*)dst, (const uint4*)src,
+                     count);
   return hipGetLastError();
 }
 
@@ -387,7 +464,7 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_integrate_hash, dim3(grid_for(n, kBlock * kIhPer)), dim3(kBlock), 0, s, P,
                      pos, vel, dt, keysOut, prevSortedHash, cellStart, cellEnd, n, passes,
-                     sort.histogram, sort.offsets, sort.counters, devFlags);
+                     sort.histogram, sort.counters);
   return hipGetLastError();
 }
 
@@ -425,9 +502,27 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
                        cellStart, cellEnd, n);
   else
-    hipLaunchKernelGGL(k_collide_v3, g, b, 0, s, P,
+    hipLaunchKernelGGL(k_collide_v3<false>, g, b, 0, s, P,
                        contact_d2_threshold(P.particleRadius + P.particleRadius), newVel,
-                       sortedPos, sortedVel, index, cellStart, cellEnd, n);
+                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
+                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+bool collide_can_fuse(const fc_sim_params& P, uint32_t n) {
+  return n > 0 && n < (1u << 27) && P.gridSize.x >= 4;
+}
+
+hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+                                const float4* sortedPos, const float4* sortedVel,
+                                const uint32_t* index, const uint32_t* cellStart,
+                                const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
+                                SortScratch& sort, int passes, uint32_t* devFlags) {
+  if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_collide_v3<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P,
+                     contact_d2_threshold(P.particleRadius + P.particleRadius), vel, sortedPos,
+                     sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut,
+                     passes, sort.histogram, sort.counters);
   return hipGetLastError();
 }
 

@@ -236,8 +236,6 @@ fc_status fc_context_create(int device, fc_context** out) {
                                         sizeof(uint32_t)) == hipSuccess &&
             hipMemset(s.histogram, 0, (size_t)RS_HIST_COPIES * RS_MAX_PASSES * RS_RADIX *
                                           sizeof(uint32_t)) == hipSuccess &&
-            hipMalloc(&s.offsets, (size_t)RS_MAX_PASSES * RS_RADIX * sizeof(uint32_t)) ==
-                hipSuccess &&
             hipMemset(ctx->devFlags, 0, sizeof(uint32_t)) == hipSuccess &&
             hipMemset(s.counters, 0, RSC_COUNT * sizeof(uint32_t)) == hipSuccess;
   if (!ok) {
@@ -259,7 +257,6 @@ fc_status fc_context_destroy(fc_context* ctx) {
   hipFree(s.valsB);
   hipFree(s.status);
   hipFree(s.histogram);
-  hipFree(s.offsets);
   hipFree(s.counters);
   hipFree(ctx->devFlags);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -414,6 +411,28 @@ fc_status fc_context_get_timing(fc_context* ctx, fc_stage_timing* out, int max, 
     out[i].launches = ctx->timer.stages[i].launches;
     out[i].total_ms = ctx->timer.stages[i].total_ms;
   }
+  return FC_SUCCESS;
+}
+
+fc_status fc_copy_probe(fc_context* ctx, void* dst, const void* src, size_t bytes, int iterations,
+                        float* avgMs) {
+  if (!ctx || !dst || !src || !avgMs || iterations < 1 || bytes % 16 != 0)
+    return set_error(FC_ERROR_INVALID_VALUE, "bad copy-probe arguments");
+  hipEvent_t a = nullptr, b = nullptr;
+  FC_HIP_TRY(hipEventCreate(&a));
+  FC_HIP_TRY(hipEventCreate(&b));
+  hipError_t e = launch_copy16(ctx->stream, dst, src, bytes);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+  for (int i = 0; i < iterations && e == hipSuccess; ++i)
+    e = launch_copy16(ctx->stream, dst, src, bytes);
+  if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (e != hipSuccess) return hip_error(e, "copy probe");
+  *avgMs = ms / (float)iterations;
   return FC_SUCCESS;
 }
 

@@ -52,13 +52,12 @@ constexpr int RS_RADIX_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_RADIX_BITS;
 constexpr int RS_MAX_PASSES = 4;
 constexpr int RS_MAX_HIST_BLOCKS = 512;
-constexpr int RS_HIST_COPIES = 32;  // privatised copies of the global histogram
+constexpr int RS_HIST_COPIES = 16;  // privatised copies of the global histogram
 
 // Device counters used by the sort; all self-resetting (see radix_sort.hip).
 enum : int {
-  RSC_HIST_DONE = 0,
   RSC_TILE0 = 1,      // RSC_TILE0 + pass: tile-id ticket of each pass
-  RSC_EPOCH = 8,      // bumped once per sort by the histogram kernel
+  RSC_EPOCH = 8,      // bumped once per sort by the kernel producing the keys
   RSC_COUNT = 16
 };
 
@@ -70,8 +69,7 @@ struct SortScratch {
   size_t capacity = 0;               // elements in keysA/valsA/keysB/valsB
   uint64_t* status = nullptr;        // tile x digit look-back words
   size_t statusCapacity = 0;         // in words
-  uint32_t* histogram = nullptr;     // [RS_HIST_COPIES][RS_MAX_PASSES][RS_RADIX], zero between sorts
-  uint32_t* offsets = nullptr;       // [RS_MAX_PASSES][RS_RADIX] exclusive bucket starts
+  uint32_t* histogram = nullptr;     // [RS_MAX_PASSES][RS_HIST_COPIES][RS_RADIX]
   uint32_t* counters = nullptr;      // [RSC_COUNT]
 };
 
@@ -90,10 +88,15 @@ int radix_passes(int bits, bool inPlace);
 // allocated, nothing synchronises: the call is graph-capturable.
 // histogramReady: the digit histogram of keysIn was already produced (by
 // the kernel that wrote the keys), so the histogram launch is skipped.
+// clearHash: when given, the first pass also resets to EMPTY every cell of
+// cellStart/cellEnd that starts a run in the sorted array clearHash (the
+// previous update's cells); requires a pass count >= 2 (minPasses).
 hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
                             const uint32_t* keysIn, const uint32_t* valsIn,
                             uint32_t* keysOut, uint32_t* valsOut, uint32_t n, int bits,
-                            StageTimer* timer, bool histogramReady = false);
+                            StageTimer* timer, bool histogramReady = false, int minPasses = 1,
+                            const uint32_t* clearHash = nullptr, uint32_t* cellStart = nullptr,
+                            uint32_t* cellEnd = nullptr);
 
 // ---------------------------------------------------------------------------
 // Per-stage timing with HIP events.
@@ -124,6 +127,7 @@ struct StageTimer {
 // kernel arguments (no constant-memory upload per step).
 // ---------------------------------------------------------------------------
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
+hipError_t launch_copy16(hipStream_t s, void* dst, const void* src, size_t bytes);
 hipError_t launch_integrate(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
                             float dt, uint32_t n);
 hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* hash,
@@ -146,6 +150,15 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
                           const uint32_t* cellEnd, uint32_t n);
 
 float contact_d2_threshold(float R);
+// collide + the next update's integrate/hash/histogram in one launch (see
+// dem_kernels.hip); usable when collide_can_fuse(P, n).
+bool collide_can_fuse(const fc_sim_params& P, uint32_t n);
+hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+                                const float4* sortedPos, const float4* sortedVel,
+                                const uint32_t* index, const uint32_t* cellStart,
+                                const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
+                                SortScratch& sort, int passes, uint32_t* devFlags);
+
 fc_status validate_params(const fc_sim_params& P);
 // Grows the context's sort scratch to hold n keys (synchronises first).
 fc_status ensure_sort_capacity(fc_context* ctx, size_t n);

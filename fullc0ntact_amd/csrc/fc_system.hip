@@ -111,11 +111,82 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   return FC_SUCCESS;
 }
 
+// Multi-step runs chain the updates so that each collide also performs the
+// next update's integrate + hash + histogram (launch_collide_fused) and the
+// next sort's first pass clears the previous update's cells.  One run of
+// `steps` updates is
+//   head:   [integrate+hash+hist] sort reorder [collide+next integrate]
+//   middle: sort(+clear) reorder [collide+next integrate]    x (steps - 2)
+//   tail:   sort(+clear) reorder [collide]
+// which computes exactly what `steps` separate updates compute.
+int run_passes(const fc_sim_params& P) {
+  const int p = radix_passes(key_bits_for_cells(P.numCells), false);
+  return p < 2 ? 2 : p;
+}
+
+fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
+  fc_context* ctx = sys->ctx;
+  hipStream_t st = ctx->stream;
+  const fc_sim_params& P = sys->params;
+  const uint32_t n = sys->n;
+  const int passes = run_passes(P);
+  hipError_t e = hipSuccess;
+  const uint32_t* prev = nullptr;
+  if (sys->tablesTrackHash) {
+    prev = sys->hash;
+  } else {
+    if (timer) timer->begin(st, "clear_tables");
+    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells);
+    if (e == hipSuccess) e = launch_fill_u32(st, sys->cellEnd, CELL_EMPTY, P.numCells);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "clear cell tables");
+  }
+  if (timer) timer->begin(st, "integrate_hash");
+  e = launch_integrate_hash(st, P, sys->pos, sys->vel, dt, sys->keys, prev, sys->cellStart,
+                            sys->cellEnd, n, ctx->sort, passes, ctx->devFlags);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "integrate_hash");
+  sys->tablesTrackHash = true;
+  return FC_SUCCESS;
+}
+
+// sort (clearing the previous cells in its first pass unless the head just
+// did) + reorder + collide, fused with the next integrate unless `last`.
+fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last,
+                           StageTimer* timer) {
+  fc_context* ctx = sys->ctx;
+  hipStream_t st = ctx->stream;
+  const fc_sim_params& P = sys->params;
+  const uint32_t n = sys->n;
+  const int passes = run_passes(P);
+  hipError_t e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash,
+                                  sys->index, n, key_bits_for_cells(P.numCells), timer, true,
+                                  passes, clearInSort ? sys->hash : nullptr, sys->cellStart,
+                                  sys->cellEnd);
+  if (e != hipSuccess) return hip_error(e, "sort");
+  if (timer) timer->begin(st, "reorder");
+  e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,
+                     sys->index, sys->pos, sys->vel, n, P.numCells);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "reorder");
+  if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
+  if (last)
+    e = launch_collide(st, P, sys->vel, sys->sortedPos, sys->sortedVel, sys->index,
+                       sys->cellStart, sys->cellEnd, n);
+  else
+    e = launch_collide_fused(st, P, sys->pos, sys->vel, sys->sortedPos, sys->sortedVel,
+                             sys->index, sys->cellStart, sys->cellEnd, n, dt, sys->keys,
+                             ctx->sort, passes, ctx->devFlags);
+  if (timer) timer->end(st);
+  if (e != hipSuccess) return hip_error(e, "collide");
+  return FC_SUCCESS;
+}
+
 fc_status capture_graph(fc_system* sys, float dt) {
   drop_graph(sys);
   hipStream_t st = sys->ctx->stream;
   FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  fc_status s = enqueue_update(sys, dt, nullptr);
+  fc_status s = enqueue_run_body(sys, dt, /*clearInSort=*/true, /*last=*/false, nullptr);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (s != FC_SUCCESS) {
@@ -285,21 +356,28 @@ fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
   if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
   if (steps == 0) return FC_SUCCESS;
   fc_context* ctx = sys->ctx;
-  if (ctx->timer.enabled || sys->n == 0) {
+  if (steps == 1 || !collide_can_fuse(sys->params, sys->n)) {
     for (uint32_t i = 0; i < steps; ++i) FC_TRY(fc_system_update(sys, deltaTime));
     return FC_SUCCESS;
   }
-  if (!sys->tablesTrackHash) {
-    FC_TRY(enqueue_update(sys, deltaTime, nullptr));
-    --steps;
+  StageTimer* timer = ctx->timer.enabled ? &ctx->timer : nullptr;
+  FC_TRY(enqueue_run_head(sys, deltaTime, timer));
+  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/false, /*last=*/false, timer));
+  const uint32_t middle = steps - 2;
+  if (middle > 0) {
+    if (timer) {
+      for (uint32_t i = 0; i < middle; ++i)
+        FC_TRY(enqueue_run_body(sys, deltaTime, true, false, timer));
+    } else {
+      if (!sys->graphExec || sys->graphDt != deltaTime ||
+          std::memcmp(&sys->graphParams, &sys->params, sizeof(fc_sim_params)) != 0) {
+        FC_TRY(capture_graph(sys, deltaTime));
+      }
+      for (uint32_t i = 0; i < middle; ++i)
+        FC_HIP_TRY(hipGraphLaunch(sys->graphExec, ctx->stream));
+    }
   }
-  if (steps == 0) return FC_SUCCESS;
-  if (!sys->graphExec || sys->graphDt != deltaTime ||
-      std::memcmp(&sys->graphParams, &sys->params, sizeof(fc_sim_params)) != 0) {
-    FC_TRY(capture_graph(sys, deltaTime));
-  }
-  for (uint32_t i = 0; i < steps; ++i)
-    FC_HIP_TRY(hipGraphLaunch(sys->graphExec, ctx->stream));
+  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/true, /*last=*/true, timer));
   return FC_SUCCESS;
 }
 

@@ -5,11 +5,10 @@
 // lexicographic (hash, original index) order.
 //
 // Structure (one histogram launch + one launch per 8-bit digit):
-//   k_rs_histogram  every block counts all digits of its tiles into LDS with
-//                   wave-aggregated adds and adds them to a global
-//                   histogram; the last block to finish (ticket counter)
-//                   turns it into exclusive bucket offsets per pass, zeroes
-//                   it for the next sort and bumps the sort epoch.
+//   digit histograms flow from the kernel that writes the keys (digit 0)
+//                   and from each pass (digit p+1) to the next pass, see
+//                   radix_common.hpp; k_rs_histogram produces digit 0 when
+//                   the keys come from the caller.
 //   k_rs_scatter    one 4096-key tile per block, tiles taken in launch order
 //                   from a ticket counter.  Each wave ranks its 1024 keys
 //                   with ballot-based digit matching into a wave-private LDS
@@ -59,14 +58,13 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
   return peers;
 }
 
-// Standalone digit histogram (used when the keys come from the caller).
+// Standalone digit-0 histogram (used when the keys come from the caller).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_histogram(
     const uint32_t* __restrict__ keys, uint32_t n, int passes, int bits,
-    uint32_t* __restrict__ gHist, uint32_t* __restrict__ offsets, uint32_t* counters,
-    uint32_t* devFlags) {
-  __shared__ uint32_t s_hist[RS_MAX_PASSES * RS_RADIX];
-  __shared__ uint32_t s_scratch[RS_RADIX / 64];
-  for (int i = threadIdx.x; i < RS_MAX_PASSES * RS_RADIX; i += RS_THREADS) s_hist[i] = 0u;
+    uint32_t* __restrict__ gHist, uint32_t* counters, uint32_t* devFlags) {
+  __shared__ uint32_t s_hist[RS_RADIX];
+  for (int i = threadIdx.x; i < RS_RADIX; i += RS_THREADS) s_hist[i] = 0u;
+  histogram_producer_prologue(gHist, passes, counters);
   __syncthreads();
   const uint32_t highMask = bits >= 32 ? 0u : ~((1u << bits) - 1u);
   uint32_t rangeErr = 0u;
@@ -82,25 +80,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_histogram(
     for (int i = 0; i < RS_ITEMS; ++i) {
       const uint32_t idx = tile * RS_TILE + i * RS_THREADS + threadIdx.x;
       if (idx < n) rangeErr |= key[i] & highMask;
-      histogram_add_key(s_hist, passes, key[i], idx < n);
+      histogram_add_digit(s_hist, key[i] & (RS_RADIX - 1), idx < n);
     }
   }
   if (rangeErr) atomicOr(devFlags, DEVERR_SORT_KEY_RANGE);
   __syncthreads();
-  histogram_flush_and_finish(s_hist, passes, n, gHist, offsets, counters, devFlags, s_scratch);
+  histogram_flush(s_hist, gHist, 0);
 }
 
 // One LSD pass over one 4096-key tile per block (16 waves x 4 keys each).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int pass,
-    const uint32_t* __restrict__ offsets, uint64_t* status, uint32_t* counters,
-    uint32_t* devFlags) {
+    int passes, uint32_t* gHist, uint64_t* status, uint32_t* counters, uint32_t* devFlags,
+    const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
+    uint32_t* __restrict__ cellEnd) {
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
   __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
   __shared__ uint32_t s_tileStart[RS_RADIX];
   __shared__ uint32_t s_digitBase[RS_RADIX];
+  __shared__ uint32_t s_nextHist[RS_RADIX];
   __shared__ uint32_t s_scratch[RS_RADIX / 64];
   __shared__ uint32_t s_tile;
 
@@ -115,11 +115,36 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     s_tile = t;
   }
   for (int i = threadIdx.x; i < kWaves * RS_RADIX; i += RS_THREADS) (&s_waveHist[0][0])[i] = 0u;
+  if (threadIdx.x < (uint32_t)RS_RADIX) s_nextHist[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && pass == 1) {  // H[0] was consumed by pass 0
+    uint32_t* h = hist_buffer(gHist, 0);
+    for (int i = threadIdx.x; i < RS_HIST_COPIES * RS_RADIX; i += RS_THREADS) h[i] = 0u;
+  }
+  // Global bucket start of this thread's digit (threads 0..255).
+  const uint32_t digitTotal =
+      threadIdx.x < (uint32_t)RS_RADIX ? histogram_total(gHist, pass, threadIdx.x) : 0u;
   __syncthreads();
+  const uint32_t digitGlobal = scan256_exclusive(digitTotal, s_scratch);
   const uint32_t tile = s_tile;
   const uint32_t epoch = (counters[RSC_EPOCH] * 4u + (uint32_t)pass) & kEpochMask;
   const uint32_t tileBase = tile * RS_TILE;
   const uint32_t waveBase = tileBase + wave * kWaveKeys;
+  if (clearHash != nullptr) {
+    // Reset the previous update's cells: cell h was written iff it starts a
+    // run of the previous sorted hashes.  clearHash is only rewritten by the
+    // last pass, which runs after this one.
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = tileBase + i * RS_THREADS + threadIdx.x;
+      if (idx < n) {
+        const uint32_t h = clearHash[idx];
+        if (idx == 0 || clearHash[idx - 1] != h) {
+          cellStart[h] = CELL_EMPTY;
+          cellEnd[h] = CELL_EMPTY;
+        }
+      }
+    }
+  }
 
   uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
 #pragma unroll
@@ -129,11 +154,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     key[i] = valid ? keysIn[idx] : 0xffffffffu;
     val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
   }
-  // Rank within the wave, slice by slice in index order (stable).
+  // Rank within the wave, slice by slice in index order (stable), and count
+  // the next pass's digit.
+  const bool countNext = pass + 1 < passes;
 #pragma unroll
   for (int i = 0; i < RS_ITEMS; ++i) {
     const uint32_t idx = waveBase + i * 64u + lane;
     const bool valid = idx < n;
+    if (countNext)
+      histogram_add_digit(s_nextHist, (key[i] >> (shift + RS_RADIX_BITS)) & (RS_RADIX - 1), valid);
     const uint64_t active = __ballot(valid);
     const uint32_t d = (key[i] >> shift) & (RS_RADIX - 1);
     const uint64_t peers = match_digit(d, active);
@@ -203,7 +232,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       __hip_atomic_store(myStatus, pack_status(kFlagPrefix, epoch, prefix + tileCount),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_digitBase[d] = offsets[pass * RS_RADIX + d] + prefix - tileStart;
+    s_digitBase[d] = digitGlobal + prefix - tileStart;
   }
   __syncthreads();
 
@@ -236,6 +265,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       }
     }
   }
+  if (countNext) histogram_flush(s_nextHist, gHist, pass + 1);
 }
 
 }  // namespace
@@ -249,21 +279,24 @@ int key_bits_for_cells(uint32_t numCells) {
 }
 
 int radix_passes(int bits, bool inPlace) {
+  (void)inPlace;
+  // At least two passes: a single pass could not run in place, and the
+  // histogram buffers are recycled by the pass after the one that read them.
+  // A pass over an all-zero digit is a stable copy.
   int passes = (bits + RS_RADIX_BITS - 1) / RS_RADIX_BITS;
-  if (passes < 1) passes = 1;  // still produces the (identity) values
-  // A single pass cannot run in place; a second pass over an all-zero digit
-  // is a stable copy, so in-place one-digit sorts take two passes.
-  if (passes == 1 && inPlace) passes = 2;
-  return passes;
+  return passes < 2 ? 2 : passes;
 }
 
 hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
                             const uint32_t* keysIn, const uint32_t* valsIn, uint32_t* keysOut,
                             uint32_t* valsOut, uint32_t n, int bits, StageTimer* timer,
-                            bool histogramReady) {
+                            bool histogramReady, int minPasses, const uint32_t* clearHash,
+                            uint32_t* cellStart, uint32_t* cellEnd) {
   if (n == 0) return hipSuccess;
   const bool inPlace = keysIn == keysOut || (valsIn != nullptr && valsIn == valsOut);
-  const int passes = radix_passes(bits, inPlace);
+  int passes = radix_passes(bits, inPlace);
+  if (passes < minPasses) passes = minPasses;
+  if (clearHash != nullptr && passes < 2) return hipErrorInvalidValue;
   if (passes > RS_MAX_PASSES) return hipErrorInvalidValue;
   const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
   if (s.capacity < n || s.statusCapacity < (size_t)numTiles * RS_RADIX) return hipErrorInvalidValue;
@@ -273,7 +306,7 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
     if (histBlocks > (uint32_t)RS_MAX_HIST_BLOCKS) histBlocks = RS_MAX_HIST_BLOCKS;
     if (timer) timer->begin(stream, "sort_histogram");
     hipLaunchKernelGGL(k_rs_histogram, dim3(histBlocks), dim3(RS_THREADS), 0, stream, keysIn, n,
-                       passes, bits, s.histogram, s.offsets, s.counters, devFlags);
+                       passes, bits, s.histogram, s.counters, devFlags);
     e = hipGetLastError();
     if (timer) timer->end(stream);
     if (e != hipSuccess) return e;
@@ -300,7 +333,8 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
                                                "sort_pass3"};
     if (timer) timer->begin(stream, names[p]);
     hipLaunchKernelGGL(k_rs_scatter, dim3(numTiles), dim3(RS_THREADS), 0, stream, kin, vin, kout,
-                       vout, n, p, s.offsets, s.status, s.counters, devFlags);
+                       vout, n, p, passes, s.histogram, s.status, s.counters, devFlags,
+                       p == 0 ? clearHash : nullptr, cellStart, cellEnd);
     e = hipGetLastError();
     if (timer) timer->end(stream);
     if (e != hipSuccess) return e;

@@ -222,6 +222,18 @@ fc_status fc_context_reset_timing(fc_context* ctx);
  * stages with data. */
 fc_status fc_context_get_timing(fc_context* ctx, fc_stage_timing* out, int max, int* count);
 
+/* ------------------------------------------------------------------------- */
+/* Diagnostics.                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* No counterpart in the reference: copies `bytes` (a multiple of 16) from
+ * src to dst `iterations` times with a plain copy kernel (one 16-byte load
+ * and store per lane) on the context stream, and returns the average kernel
+ * time measured with HIP events.  Calibrates the HBM byte counters of a
+ * profiler and measures the achievable copy bandwidth of the device. */
+fc_status fc_copy_probe(fc_context* ctx, void* dst, const void* src, size_t bytes, int iterations,
+                        float* avgMs);
+
 #ifdef __cplusplus
 }
 #endif
