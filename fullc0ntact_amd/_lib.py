@@ -70,6 +70,16 @@ class SimParams(C.Structure):
 assert C.sizeof(SimParams) == 104
 
 
+class TransInfo(C.Structure):
+    """fc_trans_info = TransInfo (inshape3dcore/rigidbody.h:54-66), row-major."""
+    _fields_ = [("origin0", Float3), ("origin1", Float3), ("m2w0", C.c_float * 9),
+                ("w2m0", C.c_float * 9), ("m2w1", C.c_float * 9)]
+
+
+FC_BOUNDARY_BOX = 0
+FC_BOUNDARY_DISTMAP = 1
+
+
 class StageTiming(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -129,6 +139,19 @@ _SIGS = {
     "fc_context_reset_timing": (_st, [_vp]),
     "fc_context_get_timing": (_st, [_vp, C.POINTER(StageTiming), C.c_int, C.POINTER(C.c_int)]),
     "fc_copy_probe": (_st, [_vp, _vp, _vp, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
+    "fc_system_set_boundary": (_st, [_vp, C.c_int, _vp]),
+    "fc_distmap_create": (_st, [_vp, C.POINTER(C.c_int32), C.c_float, C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "fc_distmap_build": (_st, [_vp, _vp, _u32, _vp, _u32, C.POINTER(C.c_int32), C.c_float,
+                               C.POINTER(C.c_float), C.POINTER(_vp)]),
+    "fc_distmap_destroy": (_st, [_vp]),
+    "fc_distmap_info": (_st, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "fc_distmap_download": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "fc_distmap_query": (_st, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "fc_distmap_eval": (_st, [_vp, _vp, _vp, _u32, C.POINTER(TransInfo), _vp, _vp, _vp]),
+    "fc_integrate_system_distmap": (_st, [_vp, _vp, _vp, _vp, C.c_float, _u32]),
+    "fc_mesh_signed_distance": (_st, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp]),
 }
 
 

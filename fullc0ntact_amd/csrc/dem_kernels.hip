@@ -8,6 +8,7 @@
 #include "fc_internal.hpp"
 #include "fc_device.hpp"
 #include "radix_common.hpp"
+#include "distmap.hpp"
 
 #include <cmath>
 
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_calc_hash(fc_sim_params P, uint32_t*
 constexpr int kIhPer = 4;
 
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(
-    fc_sim_params P, float4* __restrict__ pos, float4* __restrict__ vel, float dt,
+    fc_sim_params P, BoundaryArg B, float4* __restrict__ pos, float4* __restrict__ vel, float dt,
     uint32_t* __restrict__ keysOut, const uint32_t* __restrict__ prevSortedHash,
     uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t n, int passes,
     uint32_t* gHist, uint32_t* counters) {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     const float4 p4 = pos[i];
     const float4 v4 = vel[i];
     v3f p = xyz(p4), v = xyz(v4);
-    integrate_box(P, dt, p, v);
+    integrate_with(P, B, dt, p, v);
     pos[i] = make_float4(p.x, p.y, p.z, p4.w);
     vel[i] = make_float4(v.x, v.y, v.z, v4.w);
     key = hash_of(P, p.x, p.y, p.z);
@@ -382,7 +383,7 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
 // stages would write.
 template <bool kFuse>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
-    fc_sim_params P, float contactD2, float4* __restrict__ newVel,
+    fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
     const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
     const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
       newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
     } else {
       v3f p = xyz(p4), v = nv;
-      integrate_box(P, dt, p, v);
+      integrate_with(P, B, dt, p, v);
       pos[orig] = make_float4(p.x, p.y, p.z, p4.w);
       newVel[orig] = make_float4(v.x, v.y, v.z, 0.0f);
       key = hash_of(P, p.x, p.y, p.z);
@@ -457,12 +458,14 @@ hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* has
   return hipGetLastError();
 }
 
-hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
-                                 float dt, uint32_t* keysOut, const uint32_t* prevSortedHash,
-                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
-                                 SortScratch& sort, int passes, uint32_t* devFlags) {
+hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
+                                 float4* pos, float4* vel, float dt, uint32_t* keysOut,
+                                 const uint32_t* prevSortedHash, uint32_t* cellStart,
+                                 uint32_t* cellEnd, uint32_t n, SortScratch& sort, int passes,
+                                 uint32_t* devFlags) {
+  (void)devFlags;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_integrate_hash, dim3(grid_for(n, kBlock * kIhPer)), dim3(kBlock), 0, s, P,
+  hipLaunchKernelGGL(k_integrate_hash, dim3(grid_for(n, kBlock * kIhPer)), dim3(kBlock), 0, s, P, B,
                      pos, vel, dt, keysOut, prevSortedHash, cellStart, cellEnd, n, passes,
                      sort.histogram, sort.counters);
   return hipGetLastError();
@@ -502,7 +505,7 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
                        cellStart, cellEnd, n);
   else
-    hipLaunchKernelGGL(k_collide_v3<false>, g, b, 0, s, P,
+    hipLaunchKernelGGL(k_collide_v3<false>, g, b, 0, s, P, BoundaryArg{},
                        contact_d2_threshold(P.particleRadius + P.particleRadius), newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
                        nullptr, 0.0f, nullptr, 0, nullptr, nullptr);
@@ -513,13 +516,14 @@ bool collide_can_fuse(const fc_sim_params& P, uint32_t n) {
   return n > 0 && n < (1u << 27) && P.gridSize.x >= 4;
 }
 
-hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
+                                float4* pos, float4* vel,
                                 const float4* sortedPos, const float4* sortedVel,
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_collide_v3<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P,
+  hipLaunchKernelGGL(k_collide_v3<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, B,
                      contact_d2_threshold(P.particleRadius + P.particleRadius), vel, sortedPos,
                      sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut,
                      passes, sort.histogram, sort.counters);

@@ -1,0 +1,601 @@
+// Distance-map boundary on gfx950: upload of a reference-layout map, map
+// queries (plain and rigidly transformed), and the exact point-mesh builder.
+#include "fc_internal.hpp"
+#include "distmap.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace fc {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n, int block) {
+  size_t g = (n + (size_t)block - 1) / (size_t)block;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+// Batched queryMap over float[3] points (the reference's vector3 arrays).
+__global__ __launch_bounds__(kBlock) void k_distmap_query(DistMapView m,
+                                                          const float* __restrict__ pts,
+                                                          uint32_t n, float* __restrict__ dist,
+                                                          float* __restrict__ cp,
+                                                          float* __restrict__ nrm) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const v3f q = make_v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  float d = 0.0f;
+  v3f c = make_v3(0.0f, 0.0f, 0.0f), nn = make_v3(0.0f, 0.0f, 0.0f);
+  distmap_query(m, q, d, c, &nn);
+  dist[i] = d;
+  if (cp) {
+    cp[3 * i] = c.x;
+    cp[3 * i + 1] = c.y;
+    cp[3 * i + 2] = c.z;
+  }
+  if (nrm) {
+    nrm[3 * i] = nn.x;
+    nrm[3 * i + 1] = nn.y;
+    nrm[3 * i + 2] = nn.z;
+  }
+}
+
+// Matrix3x3<T> * Vector3<T> (math/matrix3x3.h:313-321): row major,
+// accumulated from zero.
+__device__ __forceinline__ v3f mat3_mul(const float* M, v3f v) {
+  float r[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float s = 0.0f;
+    s += M[i * 3 + 0] * v.x;
+    s += M[i * 3 + 1] * v.y;
+    s += M[i * 3 + 2] * v.z;
+    r[i] = s;
+  }
+  return make_v3(r[0], r[1], r[2]);
+}
+
+// eval_distmap_kernel (difi.cu:427-474).
+__global__ __launch_bounds__(kBlock) void k_distmap_eval(DistMapView m, fc_trans_info T,
+                                                         const float* __restrict__ v, uint32_t n,
+                                                         float* __restrict__ cps,
+                                                         float* __restrict__ normals,
+                                                         float* __restrict__ distance) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const v3f p = make_v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  v3f qw = mat3_mul(T.m2w1, p);
+  qw = make_v3(qw.x + T.origin1.x, qw.y + T.origin1.y, qw.z + T.origin1.z);
+  const v3f q = mat3_mul(T.w2m0, make_v3(qw.x - T.origin0.x, qw.y - T.origin0.y, qw.z - T.origin0.z));
+  if (!(m.bmin[0] <= q.x && q.x <= m.bmax[0] && m.bmin[1] <= q.y && q.y <= m.bmax[1] &&
+        m.bmin[2] <= q.z && q.z <= m.bmax[2])) {
+    distance[i] = 1000.0f;
+    return;
+  }
+  float d = 1000.0f;
+  v3f cp = make_v3(0.0f, 0.0f, 0.0f);
+  distmap_query(m, q, d, cp, nullptr);
+  v3f c0 = mat3_mul(T.m2w0, cp);
+  c0 = make_v3(c0.x + T.origin0.x, c0.y + T.origin0.y, c0.z + T.origin0.z);
+  // Vector3::normalize (math/vector3.h:159-167): scale by (float)(1.0/|n|).
+  const v3f nn = make_v3(c0.x - qw.x, c0.y - qw.y, c0.z - qw.z);
+  const float mag = sqrtf(dot3(nn, nn));
+  const float invMag = (float)(1.0 / (double)mag);
+  normals[3 * i] = nn.x * invMag;
+  normals[3 * i + 1] = nn.y * invMag;
+  normals[3 * i + 2] = nn.z * invMag;
+  cps[3 * i] = (c0.x + qw.x) * 0.5f;
+  cps[3 * i + 1] = (c0.y + qw.y) * 0.5f;
+  cps[3 * i + 2] = (c0.z + qw.z) * 0.5f;
+  distance[i] = d;
+}
+
+// integrateSystem with the distance-map boundary on caller arrays.
+__global__ __launch_bounds__(kBlock) void k_integrate_distmap(fc_sim_params P, DistMapView m,
+                                                              float4* __restrict__ pos,
+                                                              float4* __restrict__ vel, float dt,
+                                                              uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 p4 = pos[i], v4 = vel[i];
+  v3f p = xyz(p4), v = xyz(v4);
+  integrate_distmap(P, m, dt, p, v);
+  pos[i] = make_float4(p.x, p.y, p.z, p4.w);
+  vel[i] = make_float4(v.x, v.y, v.z, v4.w);
+}
+
+// ---------------------------------------------------------------------------
+// Builder: exact signed distance of every lattice vertex to a closed mesh.
+// ---------------------------------------------------------------------------
+
+// Squared distance from q to triangle (B, V1, V2) and the closest point, by
+// the region method of D. Eberly, "Distance Between Point and Triangle in
+// 3D" (Geometric Tools), in binary64.  (The reference's variant,
+// distance/distancetriangle.cpp:55-190, departs from it in regions 4 and 6.)
+__device__ __forceinline__ double point_triangle_dist2(const double q[3], const double* B,
+                                                       const double* V1, const double* V2,
+                                                       double cp[3]) {
+  double E0[3], E1[3], D[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    E0[k] = V1[k] - B[k];
+    E1[k] = V2[k] - B[k];
+    D[k] = B[k] - q[k];
+  }
+  const double a = E0[0] * E0[0] + E0[1] * E0[1] + E0[2] * E0[2];
+  const double b = E0[0] * E1[0] + E0[1] * E1[1] + E0[2] * E1[2];
+  const double c = E1[0] * E1[0] + E1[1] * E1[1] + E1[2] * E1[2];
+  const double d = E0[0] * D[0] + E0[1] * D[1] + E0[2] * D[2];
+  const double e = E1[0] * D[0] + E1[1] * D[1] + E1[2] * D[2];
+  const double det = a * c - b * b;
+  double s = b * e - c * d, t = b * d - a * e;
+  if (s + t <= det) {
+    if (s < 0.0) {
+      if (t < 0.0) {  // region 4
+        if (d < 0.0) {
+          t = 0.0;
+          s = (-d >= a) ? 1.0 : -d / a;
+        } else {
+          s = 0.0;
+          t = (e >= 0.0) ? 0.0 : ((-e >= c) ? 1.0 : -e / c);
+        }
+      } else {  // region 3
+        s = 0.0;
+        t = (e >= 0.0) ? 0.0 : ((-e >= c) ? 1.0 : -e / c);
+      }
+    } else if (t < 0.0) {  // region 5
+      t = 0.0;
+      s = (d >= 0.0) ? 0.0 : ((-d >= a) ? 1.0 : -d / a);
+    } else {  // region 0
+      const double inv = 1.0 / det;
+      s *= inv;
+      t *= inv;
+    }
+  } else {
+    if (s < 0.0) {  // region 2
+      const double tmp0 = b + d, tmp1 = c + e;
+      if (tmp1 > tmp0) {
+        const double numer = tmp1 - tmp0, denom = a - 2.0 * b + c;
+        s = (numer >= denom) ? 1.0 : numer / denom;
+        t = 1.0 - s;
+      } else {
+        s = 0.0;
+        t = (tmp1 <= 0.0) ? 1.0 : ((e >= 0.0) ? 0.0 : -e / c);
+      }
+    } else if (t < 0.0) {  // region 6
+      const double tmp0 = b + e, tmp1 = a + d;
+      if (tmp1 > tmp0) {
+        const double numer = tmp1 - tmp0, denom = a - 2.0 * b + c;
+        t = (numer >= denom) ? 1.0 : numer / denom;
+        s = 1.0 - t;
+      } else {
+        t = 0.0;
+        s = (tmp1 <= 0.0) ? 1.0 : ((d >= 0.0) ? 0.0 : -d / a);
+      }
+    } else {  // region 1
+      const double numer = (c + e) - (b + d);
+      if (numer <= 0.0) {
+        s = 0.0;
+      } else {
+        const double denom = a - 2.0 * b + c;
+        s = (numer >= denom) ? 1.0 : numer / denom;
+      }
+      t = 1.0 - s;
+    }
+  }
+  double r2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    cp[k] = (B[k] + s * E0[k]) + t * E1[k];
+    const double diff = cp[k] - q[k];
+    r2 += diff * diff;
+  }
+  return r2;
+}
+
+// Solid angle of a triangle seen from q (Van Oosterom & Strackee 1983).
+__device__ __forceinline__ double solid_angle(const double q[3], const double* A, const double* B,
+                                              const double* C) {
+  double a[3], b[3], c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    a[k] = A[k] - q[k];
+    b[k] = B[k] - q[k];
+    c[k] = C[k] - q[k];
+  }
+  const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  const double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                     a[2] * (b[0] * c[1] - b[1] * c[0]);
+  const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  const double bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+  const double ca = c[0] * a[0] + c[1] * a[1] + c[2] * a[2];
+  const double den = la * lb * lc + ab * lc + bc * la + ca * lb;
+  return 2.0 * atan2(det, den);
+}
+
+constexpr int kTriTile = 128;
+
+// One lane per query point (lattice vertex or free point); triangles are
+// streamed through LDS in tiles that every lane of the block reads
+// (broadcast reads, no bank conflicts).  Minimum over triangles in order
+// (first minimum wins), sign from the generalized winding number.
+template <bool kLattice>
+__global__ __launch_bounds__(kBlock) void k_mesh_sdf(
+    const double* __restrict__ verts, uint32_t nv, const uint32_t* __restrict__ tris, uint32_t nt,
+    uint64_t npts, const double* __restrict__ pts, int cx, int cy, double bx, double by,
+    double bz, double cs, float4* __restrict__ cpd, float4* __restrict__ nrm,
+    double* __restrict__ sdfOut, double* __restrict__ cpOut) {
+  __shared__ double s_tri[kTriTile][9];
+  const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = v < npts;
+  double q[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    if (kLattice) {
+      const uint64_t ix = (uint64_t)cx + 1, islice = ix * ((uint64_t)cy + 1);
+      const uint64_t iz = v / islice, rem = v - iz * islice, iy = rem / ix, ixx = rem - iy * ix;
+      q[0] = bx + (double)ixx * cs;
+      q[1] = by + (double)iy * cs;
+      q[2] = bz + (double)iz * cs;
+    } else {
+      q[0] = pts[3 * v];
+      q[1] = pts[3 * v + 1];
+      q[2] = pts[3 * v + 2];
+    }
+  }
+  double best = DBL_MAX, bcp[3] = {0.0, 0.0, 0.0}, w = 0.0;
+  for (uint32_t t0 = 0; t0 < nt; t0 += kTriTile) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTriTile * 3; k += kBlock) {
+      const uint32_t t = t0 + k / 3;
+      if (t < nt) {
+        uint32_t vi = tris[3 * t + k % 3];
+        if (vi >= nv) vi = nv - 1;  // out-of-range indices must not read past the mesh
+        s_tri[k / 3][3 * (k % 3)] = verts[3 * vi];
+        s_tri[k / 3][3 * (k % 3) + 1] = verts[3 * vi + 1];
+        s_tri[k / 3][3 * (k % 3) + 2] = verts[3 * vi + 2];
+      }
+    }
+    __syncthreads();
+    if (!valid) continue;
+    const uint32_t cnt = (nt - t0) < (uint32_t)kTriTile ? (nt - t0) : (uint32_t)kTriTile;
+    for (uint32_t t = 0; t < cnt; ++t) {
+      const double* T = s_tri[t];
+      double c3[3];
+      const double d2 = point_triangle_dist2(q, T, T + 3, T + 6, c3);
+      if (d2 < best) {
+        best = d2;
+        bcp[0] = c3[0];
+        bcp[1] = c3[1];
+        bcp[2] = c3[2];
+      }
+      w += solid_angle(q, T, T + 3, T + 6);
+    }
+  }
+  if (!valid) return;
+  w /= 4.0 * 3.14159265358979323846;
+  const double dist = sqrt(best);
+  const double sd = (w > 0.5) ? -dist : dist;
+  if (kLattice) {
+    cpd[v] = make_float4((float)bcp[0], (float)bcp[1], (float)bcp[2], (float)sd);
+    nrm[v] = make_float4((float)(q[0] - bcp[0]), (float)(q[1] - bcp[1]), (float)(q[2] - bcp[2]),
+                         0.0f);
+  } else {
+    sdfOut[v] = sd;
+    if (cpOut) {
+      cpOut[3 * v] = bcp[0];
+      cpOut[3 * v + 1] = bcp[1];
+      cpOut[3 * v + 2] = bcp[2];
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
+                                    float4* pos, float4* vel, float dt, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_integrate_distmap, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, m,
+                     pos, vel, dt, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_sdf_points(hipStream_t s, const double* verts, uint32_t nv,
+                                  const uint32_t* tris,
+                                  uint32_t nt, const double* pts, uint64_t np, double* sdf,
+                                  double* cp) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mesh_sdf<false>, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, s, verts, nv,
+                     tris, nt, np, pts, 0, 0, 0.0, 0.0, 0.0, 0.0, nullptr, nullptr, sdf, cp);
+  return hipGetLastError();
+}
+
+}  // namespace fc
+
+using namespace fc;
+
+struct fc_distmap {
+  fc_context* ctx = nullptr;
+  int cells[3] = {0, 0, 0};
+  float cellSize = 0.0f;
+  float bmin[3] = {0, 0, 0};
+  float bmax[3] = {0, 0, 0};
+  float* xs = nullptr;
+  float* ys = nullptr;
+  float* zs = nullptr;
+  float4* cpd = nullptr;
+  float4* nrm = nullptr;
+  size_t V = 0;
+};
+
+namespace fc {
+
+DistMapView distmap_view(const fc_distmap* m) {
+  DistMapView v;
+  v.cx = m->cells[0];
+  v.cy = m->cells[1];
+  v.cz = m->cells[2];
+  v.cellSize = m->cellSize;
+  for (int a = 0; a < 3; ++a) {
+    v.bmin[a] = m->bmin[a];
+    v.bmax[a] = m->bmax[a];
+  }
+  v.xs = m->xs;
+  v.ys = m->ys;
+  v.zs = m->zs;
+  v.cpd = m->cpd;
+  v.nrm = m->nrm;
+  return v;
+}
+
+}  // namespace fc
+
+namespace {
+
+fc_status alloc_map(fc_context* ctx, const int32_t cells[3], fc_distmap** out) {
+  for (int a = 0; a < 3; ++a)
+    if (cells[a] < 1) return set_error(FC_ERROR_INVALID_VALUE, "cells must be >= 1 per axis");
+  const size_t V = (size_t)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);
+  if (V > ((size_t)1 << 33)) return set_error(FC_ERROR_CAPACITY, "distance map too large");
+  fc_distmap* m = new fc_distmap();
+  m->ctx = ctx;
+  for (int a = 0; a < 3; ++a) m->cells[a] = cells[a];
+  m->V = V;
+  hipError_t e = hipMalloc(&m->xs, (cells[0] + 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&m->ys, (cells[1] + 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&m->zs, (cells[2] + 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&m->cpd, V * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&m->nrm, V * sizeof(float4));
+  if (e != hipSuccess) {
+    fc_distmap_destroy(m);
+    return hip_error(e, "distance map allocation");
+  }
+  *out = m;
+  return FC_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+fc_status fc_distmap_create(fc_context* ctx, const int32_t cells[3], float cellSize,
+                            const float bmin[3], const float bmax[3], const float* vertexCoords,
+                            const float* normals, const float* contactPoints,
+                            const float* distance, fc_distmap** out) {
+  if (!ctx || !cells || !bmin || !bmax || !vertexCoords || !normals || !contactPoints ||
+      !distance || !out)
+    return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *out = nullptr;
+  fc_distmap* m = nullptr;
+  FC_TRY(alloc_map(ctx, cells, &m));
+  const int cx = cells[0], cy = cells[1], cz = cells[2];
+  const size_t ix = cx + 1, islice = ix * (cy + 1), V = m->V;
+  std::vector<float> xs(cx + 1), ys(cy + 1), zs(cz + 1);
+  for (int i = 0; i <= cx; ++i) xs[i] = vertexCoords[3 * i];
+  for (int i = 0; i <= cy; ++i) ys[i] = vertexCoords[3 * (i * ix) + 1];
+  for (int i = 0; i <= cz; ++i) zs[i] = vertexCoords[3 * (i * islice) + 2];
+  std::vector<float4> cpd(V), nrm(V);
+  for (size_t v = 0; v < V; ++v) {
+    const size_t z = v / islice, y = (v % islice) / ix, x = v % ix;
+    if (vertexCoords[3 * v] != xs[x] || vertexCoords[3 * v + 1] != ys[y] ||
+        vertexCoords[3 * v + 2] != zs[z]) {
+      fc_distmap_destroy(m);
+      return set_error(FC_ERROR_INVALID_VALUE,
+                       "vertex coordinates are not a tensor-product lattice");
+    }
+    cpd[v] = make_float4(contactPoints[3 * v], contactPoints[3 * v + 1],
+                         contactPoints[3 * v + 2], distance[v]);
+    nrm[v] = make_float4(normals[3 * v], normals[3 * v + 1], normals[3 * v + 2], 0.0f);
+  }
+  m->cellSize = cellSize;
+  for (int a = 0; a < 3; ++a) {
+    m->bmin[a] = bmin[a];
+    m->bmax[a] = bmax[a];
+  }
+  hipStream_t st = ctx->stream;
+  hipError_t e = hipMemcpyAsync(m->xs, xs.data(), xs.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->ys, ys.data(), ys.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->zs, zs.data(), zs.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->cpd, cpd.data(), V * 16, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->nrm, nrm.data(), V * 16, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    fc_distmap_destroy(m);
+    return hip_error(e, "distance map upload");
+  }
+  *out = m;
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_build(fc_context* ctx, const double* vertices, uint32_t numVertices,
+                           const uint32_t* triangles, uint32_t numTriangles,
+                           const int32_t cells[3], float cellSize, const float bmin[3],
+                           fc_distmap** out) {
+  if (!ctx || !vertices || !triangles || !cells || !bmin || !out)
+    return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (numTriangles == 0 || numVertices == 0)
+    return set_error(FC_ERROR_INVALID_VALUE, "empty mesh");
+  if (!(cellSize > 0.0f)) return set_error(FC_ERROR_INVALID_VALUE, "cellSize must be positive");
+  for (uint64_t i = 0; i < 3ull * numTriangles; ++i)
+    if (triangles[i] >= numVertices)
+      return set_error(FC_ERROR_INVALID_VALUE, "triangle index out of range");
+  *out = nullptr;
+  fc_distmap* m = nullptr;
+  FC_TRY(alloc_map(ctx, cells, &m));
+  const double cs = cellSize, b[3] = {bmin[0], bmin[1], bmin[2]};
+  std::vector<float> ax[3];
+  for (int a = 0; a < 3; ++a) {
+    ax[a].resize(cells[a] + 1);
+    for (int i = 0; i <= cells[a]; ++i) ax[a][i] = (float)(b[a] + (double)i * cs);
+    m->bmin[a] = bmin[a];
+    m->bmax[a] = (float)(b[a] + (double)cells[a] * cs);
+  }
+  m->cellSize = cellSize;
+  hipStream_t st = ctx->stream;
+  double* dv = nullptr;
+  uint32_t* dt = nullptr;
+  hipError_t e = hipMalloc(&dv, (size_t)numVertices * 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dt, (size_t)numTriangles * 3 * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dv, vertices, (size_t)numVertices * 24, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dt, triangles, (size_t)numTriangles * 12, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->xs, ax[0].data(), ax[0].size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->ys, ax[1].data(), ax[1].size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->zs, ax[2].data(), ax[2].size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mesh_sdf<true>, dim3(grid_for(m->V, kBlock)), dim3(kBlock), 0, st, dv,
+                       numVertices, dt, numTriangles, (uint64_t)m->V, nullptr, cells[0], cells[1], b[0], b[1],
+                       b[2], cs, m->cpd, m->nrm, nullptr, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(dv);
+  hipFree(dt);
+  if (e != hipSuccess) {
+    fc_distmap_destroy(m);
+    return hip_error(e, "distance map build");
+  }
+  *out = m;
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_destroy(fc_distmap* m) {
+  if (!m) return FC_SUCCESS;
+  if (m->ctx && m->ctx->stream) hipStreamSynchronize(m->ctx->stream);
+  hipFree(m->xs);
+  hipFree(m->ys);
+  hipFree(m->zs);
+  hipFree(m->cpd);
+  hipFree(m->nrm);
+  delete m;
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_info(const fc_distmap* m, int32_t cells[3], float* cellSize, float bmin[3],
+                          float bmax[3]) {
+  if (!m) return set_error(FC_ERROR_INVALID_VALUE, "map is NULL");
+  for (int a = 0; a < 3; ++a) {
+    if (cells) cells[a] = m->cells[a];
+    if (bmin) bmin[a] = m->bmin[a];
+    if (bmax) bmax[a] = m->bmax[a];
+  }
+  if (cellSize) *cellSize = m->cellSize;
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_download(fc_context* ctx, const fc_distmap* m, float* vertexCoords,
+                              float* normals, float* contactPoints, float* distance) {
+  if (!ctx || !m) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  const size_t V = m->V, ix = m->cells[0] + 1, islice = ix * (m->cells[1] + 1);
+  std::vector<float4> cpd(V), nrm(V);
+  std::vector<float> ax[3];
+  const float* src[3] = {m->xs, m->ys, m->zs};
+  hipStream_t st = ctx->stream;
+  hipError_t e = hipSuccess;
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    ax[a].resize(m->cells[a] + 1);
+    e = hipMemcpyAsync(ax[a].data(), src[a], ax[a].size() * 4, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(cpd.data(), m->cpd, V * 16, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(nrm.data(), m->nrm, V * 16, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_error(e, "distance map download");
+  for (size_t v = 0; v < V; ++v) {
+    const size_t z = v / islice, y = (v % islice) / ix, x = v % ix;
+    if (vertexCoords) {
+      vertexCoords[3 * v] = ax[0][x];
+      vertexCoords[3 * v + 1] = ax[1][y];
+      vertexCoords[3 * v + 2] = ax[2][z];
+    }
+    if (contactPoints) {
+      contactPoints[3 * v] = cpd[v].x;
+      contactPoints[3 * v + 1] = cpd[v].y;
+      contactPoints[3 * v + 2] = cpd[v].z;
+    }
+    if (distance) distance[v] = cpd[v].w;
+    if (normals) {
+      normals[3 * v] = nrm[v].x;
+      normals[3 * v + 1] = nrm[v].y;
+      normals[3 * v + 2] = nrm[v].z;
+    }
+  }
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_query(fc_context* ctx, const fc_distmap* m, const float* points, uint32_t n,
+                           float* dist, float* cp, float* normal) {
+  if (!ctx || !m) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (n == 0) return FC_SUCCESS;
+  if (!points || !dist) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  hipLaunchKernelGGL(k_distmap_query, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                     distmap_view(m), points, n, dist, cp, normal);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(e, "distmap query");
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_eval(fc_context* ctx, const fc_distmap* m, const float* v, uint32_t n,
+                          const fc_trans_info* info, float* cps, float* normals,
+                          float* distance) {
+  if (!ctx || !m || !info) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (n == 0) return FC_SUCCESS;
+  if (!v || !cps || !normals || !distance) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  hipLaunchKernelGGL(k_distmap_eval, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, ctx->stream,
+                     distmap_view(m), *info, v, n, cps, normals, distance);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(e, "eval_distmap");
+  return FC_SUCCESS;
+}
+
+fc_status fc_integrate_system_distmap(fc_context* ctx, const fc_distmap* m, float* pos, float* vel,
+                                      float deltaTime, uint32_t numParticles) {
+  if (!ctx || !m) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (numParticles == 0) return FC_SUCCESS;
+  if (!pos || !vel) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  hipError_t e = launch_integrate_distmap(ctx->stream, ctx->params, distmap_view(m), (float4*)pos,
+                                          (float4*)vel, deltaTime, numParticles);
+  if (e != hipSuccess) return hip_error(e, "integrate (distance map)");
+  return FC_SUCCESS;
+}
+
+fc_status fc_mesh_signed_distance(fc_context* ctx, const double* vertices, uint32_t numVertices,
+                                  const uint32_t* triangles, uint32_t numTriangles,
+                                  const double* points, uint32_t numPoints, double* sdf,
+                                  double* closest) {
+  if (!ctx || !vertices || !triangles || !points || !sdf)
+    return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (numPoints == 0) return FC_SUCCESS;
+  if (numTriangles == 0) return set_error(FC_ERROR_INVALID_VALUE, "empty mesh");
+  if (numVertices == 0) return set_error(FC_ERROR_INVALID_VALUE, "empty mesh");
+  hipError_t e = launch_mesh_sdf_points(ctx->stream, vertices, numVertices, triangles,
+                                        numTriangles, points, numPoints, sdf, closest);
+  if (e != hipSuccess) return hip_error(e, "mesh signed distance");
+  return FC_SUCCESS;
+}
+
+}  // extern "C"

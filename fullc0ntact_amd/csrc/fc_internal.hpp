@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fc_dem.h"
+#include "distmap.hpp"
 
 struct fc_context;
 
@@ -46,7 +47,10 @@ constexpr uint32_t CELL_EMPTY = 0xffffffffu;
 // Radix sort (radix_sort.hip).
 // ---------------------------------------------------------------------------
 constexpr int RS_THREADS = 1024;              // 16 wave64s
-constexpr int RS_ITEMS = 4;                   // keys per thread per tile
+#ifndef FC_RS_ITEMS
+#define FC_RS_ITEMS 4
+#endif
+constexpr int RS_ITEMS = FC_RS_ITEMS;         // keys per thread per tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS; // 4096 keys
 constexpr int RS_RADIX_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_RADIX_BITS;
@@ -136,10 +140,14 @@ hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* has
 // previous step's sorted hashes (prevSortedHash may be null: no clearing),
 // plus the radix sort's digit histogram of the new keys for `passes`
 // passes (written into the sort scratch, as k_rs_histogram would).
-hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
-                                 float dt, uint32_t* keysOut, const uint32_t* prevSortedHash,
-                                 uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
-                                 SortScratch& sort, int passes, uint32_t* devFlags);
+hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
+                                 float4* pos, float4* vel, float dt, uint32_t* keysOut,
+                                 const uint32_t* prevSortedHash, uint32_t* cellStart,
+                                 uint32_t* cellEnd, uint32_t n, SortScratch& sort, int passes,
+                                 uint32_t* devFlags);
+hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
+                                    float4* pos, float4* vel, float dt, uint32_t n);
+DistMapView distmap_view(const fc_distmap* m);
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
@@ -153,7 +161,8 @@ float contact_d2_threshold(float R);
 // collide + the next update's integrate/hash/histogram in one launch (see
 // dem_kernels.hip); usable when collide_can_fuse(P, n).
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n);
-hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
+hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
+                                float4* pos, float4* vel,
                                 const float4* sortedPos, const float4* sortedVel,
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
@@ -202,4 +211,8 @@ struct fc_system {
   hipGraphExec_t graphExec = nullptr;
   float graphDt = 0.0f;
   fc_sim_params graphParams{};
+  // Boundary of the integrate stage.
+  fc_boundary boundary = FC_BOUNDARY_BOX;
+  const fc_distmap* boundaryMap = nullptr;
+  fc::BoundaryArg boundary_arg() const;
 };

@@ -90,8 +90,9 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   }
   const int bits = key_bits_for_cells(P.numCells);
   if (timer) timer->begin(st, "integrate_hash");
-  e = launch_integrate_hash(st, P, sys->pos, sys->vel, dt, sys->keys, prev, sys->cellStart,
-                            sys->cellEnd, n, ctx->sort, radix_passes(bits, false), ctx->devFlags);
+  e = launch_integrate_hash(st, P, sys->boundary_arg(), sys->pos, sys->vel, dt, sys->keys, prev,
+                            sys->cellStart, sys->cellEnd, n, ctx->sort, radix_passes(bits, false),
+                            ctx->devFlags);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "integrate_hash");
   e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index, n,
@@ -142,8 +143,8 @@ fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
     if (e != hipSuccess) return hip_error(e, "clear cell tables");
   }
   if (timer) timer->begin(st, "integrate_hash");
-  e = launch_integrate_hash(st, P, sys->pos, sys->vel, dt, sys->keys, prev, sys->cellStart,
-                            sys->cellEnd, n, ctx->sort, passes, ctx->devFlags);
+  e = launch_integrate_hash(st, P, sys->boundary_arg(), sys->pos, sys->vel, dt, sys->keys, prev,
+                            sys->cellStart, sys->cellEnd, n, ctx->sort, passes, ctx->devFlags);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "integrate_hash");
   sys->tablesTrackHash = true;
@@ -174,7 +175,8 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     e = launch_collide(st, P, sys->vel, sys->sortedPos, sys->sortedVel, sys->index,
                        sys->cellStart, sys->cellEnd, n);
   else
-    e = launch_collide_fused(st, P, sys->pos, sys->vel, sys->sortedPos, sys->sortedVel,
+    e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
+                             sys->sortedVel,
                              sys->index, sys->cellStart, sys->cellEnd, n, dt, sys->keys,
                              ctx->sort, passes, ctx->devFlags);
   if (timer) timer->end(st);
@@ -211,7 +213,27 @@ fc_status capture_graph(fc_system* sys, float dt) {
 
 using namespace fc;
 
+fc::BoundaryArg fc_system::boundary_arg() const {
+  fc::BoundaryArg b{};
+  b.hasMap = (boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr) ? 1 : 0;
+  if (b.hasMap) b.map = distmap_view(boundaryMap);
+  return b;
+}
+
 extern "C" {
+
+fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_distmap* map) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  if (mode != FC_BOUNDARY_BOX && mode != FC_BOUNDARY_DISTMAP)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown boundary mode");
+  if (mode == FC_BOUNDARY_DISTMAP && !map)
+    return set_error(FC_ERROR_INVALID_VALUE, "the distance-map boundary needs a map");
+  FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  sys->boundary = mode;
+  sys->boundaryMap = mode == FC_BOUNDARY_DISTMAP ? map : nullptr;
+  drop_graph(sys);  // the captured step carries the boundary in its arguments
+  return FC_SUCCESS;
+}
 
 fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 gridSize,
                            fc_system** out) {

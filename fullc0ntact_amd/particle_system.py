@@ -83,6 +83,18 @@ class ParticleSystem:
     def sync(self) -> None:
         self.ctx.sync()
 
+    def setBoundary(self, distance_map=None) -> None:
+        """Integrate against the distance-map boundary of
+        d_integrateSystem/d_checkBoundaryDistMap (particledem_um.cu:604-638,
+        boundaries.cuh:38-89), or against the box walls of integrate_functor
+        when distance_map is None.  The map must outlive its use."""
+        if distance_map is None:
+            check(lib.fc_system_set_boundary(self.handle, _lib.FC_BOUNDARY_BOX, None))
+        else:
+            check(lib.fc_system_set_boundary(self.handle, _lib.FC_BOUNDARY_DISTMAP,
+                                             distance_map.handle))
+        self._boundary = distance_map
+
     # -- initial state ------------------------------------------------------
     def reset(self, config: int, seed: int = 1973, jitter_frac: float = 0.01,
               vel_amp: float = 0.0) -> None:

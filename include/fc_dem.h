@@ -80,6 +80,26 @@ typedef enum fc_particle_config { FC_CONFIG_RANDOM = 0, FC_CONFIG_GRID = 1 } fc_
 
 typedef struct fc_context fc_context;
 typedef struct fc_system fc_system;
+typedef struct fc_distmap fc_distmap;
+
+/* TransInfo (inshape3dcore/rigidbody.h:54-66): rigid transforms of two
+ * bodies, 3x3 matrices row major. */
+typedef struct fc_trans_info {
+  fc_float3 origin0;
+  fc_float3 origin1;
+  float m2w0[9]; /* model -> world of body 0 */
+  float w2m0[9]; /* world -> model of body 0 */
+  float m2w1[9]; /* model -> world of body 1 */
+} fc_trans_info;
+
+/* Boundary applied by the integrate stage of a particle system. */
+typedef enum fc_boundary {
+  /* Box walls at +-1 (integrate_functor, particles_kernel.cu:39-75). */
+  FC_BOUNDARY_BOX = 0,
+  /* Distance-map push-out, cylinder r = 1 around z, planes z = -1 and 10
+   * (d_integrateSystem, particledem_um.cu:604-638; boundaries.cuh:38-89). */
+  FC_BOUNDARY_DISTMAP = 1
+} fc_boundary;
 
 /* ------------------------------------------------------------------------- */
 /* Runtime, errors, memory.                                                  */
@@ -203,6 +223,65 @@ fc_status fc_system_device_arrays(fc_system* sys, float** pos, float** vel, floa
 /* dumpGrid (particleSystem.cpp:286-301) without the printf: the largest
  * cellEnd-cellStart over non-empty cells, plus the number of occupied cells. */
 fc_status fc_system_dump_grid(fc_system* sys, uint32_t* maxParticlesPerCell, uint32_t* occupiedCells);
+
+/* Boundary of the integrate stage (default FC_BOUNDARY_BOX).  The map must
+ * outlive its use by the system. */
+fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_distmap* map);
+
+/* ------------------------------------------------------------------------- */
+/* Distance maps (DistanceMap<float,gpu>, cuda_addon/distancemap.cuh:6-303;  */
+/* difi.cu).                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* DistanceMap<float,gpu>::transferData (distancemap.cuh:50-95) /
+ * allocate_dmap (difi.cu:774-835): uploads a map given in the reference's
+ * host layout: (cells+1)^3 vertices indexed (iz*(cy+1)+iy)*(cx+1)+ix, 3
+ * floats each of vertexCoords, normals and contactPoints, one distance.  The
+ * vertex coordinates must form a tensor-product lattice.  bmin/bmax are the
+ * bounding box vertices_[0] / vertices_[1]. */
+fc_status fc_distmap_create(fc_context* ctx, const int32_t cells[3], float cellSize,
+                            const float bmin[3], const float bmax[3], const float* vertexCoords,
+                            const float* normals, const float* contactPoints,
+                            const float* distance, fc_distmap** out);
+/* Builds a map of a closed, consistently oriented triangle mesh (host
+ * arrays: 3 doubles per vertex, 3 indices per triangle) on the lattice
+ * bmin + (ix,iy,iz)*cellSize, computed in binary64 and stored in binary32:
+ * signed distance (negative inside), closest point, normal = vertex - closest
+ * point (DistanceMapMesh::ComputeDistance, distancegridcgal.hpp:36-71, with
+ * the exact distance of D. Eberly's point-triangle method and the sign of the
+ * generalized winding number instead of CGAL). */
+fc_status fc_distmap_build(fc_context* ctx, const double* vertices, uint32_t numVertices,
+                           const uint32_t* triangles, uint32_t numTriangles,
+                           const int32_t cells[3], float cellSize, const float bmin[3],
+                           fc_distmap** out);
+fc_status fc_distmap_destroy(fc_distmap* map);
+fc_status fc_distmap_info(const fc_distmap* map, int32_t cells[3], float* cellSize, float bmin[3],
+                          float bmax[3]);
+/* Copies a map back in the reference's host layout (any pointer may be
+ * NULL). */
+fc_status fc_distmap_download(fc_context* ctx, const fc_distmap* map, float* vertexCoords,
+                              float* normals, float* contactPoints, float* distance);
+/* queryMap (distancemap.cuh:154-191) for n points (device arrays, 3 floats
+ * per point): distance (5.0 outside the box), closest point and normal
+ * (cp, normal may be NULL). */
+fc_status fc_distmap_query(fc_context* ctx, const fc_distmap* map, const float* points, uint32_t n,
+                           float* dist, float* cp, float* normal);
+/* eval_distmap (difi.cu:427-500) on device arrays: vertices of body 1 into
+ * body 0's model frame, query, world-space normal and midpoint contact;
+ * distance 1000 (and cps/normals untouched) outside the box. */
+fc_status fc_distmap_eval(fc_context* ctx, const fc_distmap* map, const float* v, uint32_t n,
+                          const fc_trans_info* info, float* cps, float* normals, float* distance);
+/* integrateSystem with the distance-map boundary on caller arrays (device
+ * float4 pos/vel), using the context parameters. */
+fc_status fc_integrate_system_distmap(fc_context* ctx, const fc_distmap* map, float* pos,
+                                      float* vel, float deltaTime, uint32_t numParticles);
+/* Exact signed distance (and closest point, may be NULL) of numPoints
+ * points to a closed triangle mesh, all arrays on the device (binary64,
+ * 3 per vertex/point, 3 indices per triangle; indices must be < numVertices). */
+fc_status fc_mesh_signed_distance(fc_context* ctx, const double* vertices, uint32_t numVertices,
+                                  const uint32_t* triangles, uint32_t numTriangles,
+                                  const double* points, uint32_t numPoints, double* sdf,
+                                  double* closest);
 
 /* ------------------------------------------------------------------------- */
 /* Per-stage timing with HIP events on the context stream.                   */

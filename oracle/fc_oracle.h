@@ -101,6 +101,47 @@ void orc_reset(float* pos4, float* vel4, uint32_t n, int config, uint64_t seed, 
  * out[1..3] = sum v (momentum per unit mass). */
 void orc_energy_momentum(const float* vel4, uint32_t n, double* out4);
 
+/* ---- distance map (fc_oracle_distmap.c) ---- */
+
+/* A DistanceMap<float> (grid/distancemap.h:36-143) on a tensor-product
+ * lattice: vertex (ix,iy,iz) has coordinates (xs[ix], ys[iy], zs[iz]) and
+ * index (iz*(cy+1) + iy)*(cx+1) + ix; cp and nrm hold 3 floats per vertex. */
+typedef struct orc_distmap {
+  int cells[3];
+  float cellSize;
+  float bmin[3];
+  float bmax[3];
+  const float* xs;
+  const float* ys;
+  const float* zs;
+  const float* dist;
+  const float* cp;
+  const float* nrm;
+} orc_distmap;
+
+/* TransInfo (inshape3dcore/rigidbody.h:54-66), row-major 3x3 matrices. */
+typedef struct orc_transinfo {
+  float origin0[3];
+  float origin1[3];
+  float m2w0[9];
+  float w2m0[9];
+  float m2w1[9];
+} orc_transinfo;
+
+int orc_distmap_query(const orc_distmap* m, const float q[3], float* dist, float cp[3],
+                      float nrm[3]);
+void orc_distmap_query_batch(const orc_distmap* m, const float* q, uint32_t n, float* dist,
+                             float* cp, float* nrm);
+void orc_integrate_distmap(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P,
+                           const orc_distmap* m);
+void orc_distmap_eval(const orc_distmap* m, const float* v, uint32_t n, const orc_transinfo* t,
+                      float* cps, float* normals, float* distance);
+double orc_point_triangle_dist2(const double q[3], const double B[3], const double V1[3],
+                                const double V2[3], double cp[3]);
+void orc_mesh_sdf(const double* verts, const uint32_t* tris, uint32_t nt, const double* pts,
+                  uint32_t np, double* sdf, double* cp, double* winding);
+void orc_update_distmap(orc_state* s, float dt, const orc_params* P, const orc_distmap* m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,18 @@ class OrcParams(C.Structure):
     ]
 
 
+class OrcDistMap(C.Structure):
+    _fields_ = [("cells", C.c_int * 3), ("cellSize", C.c_float), ("bmin", C.c_float * 3),
+                ("bmax", C.c_float * 3), ("xs", C.c_void_p), ("ys", C.c_void_p),
+                ("zs", C.c_void_p), ("dist", C.c_void_p), ("cp", C.c_void_p),
+                ("nrm", C.c_void_p)]
+
+
+class OrcTransInfo(C.Structure):
+    _fields_ = [("origin0", C.c_float * 3), ("origin1", C.c_float * 3), ("m2w0", C.c_float * 9),
+                ("w2m0", C.c_float * 9), ("m2w1", C.c_float * 9)]
+
+
 class OrcState(C.Structure):
     _fields_ = [("pos4", C.c_void_p), ("vel4", C.c_void_p), ("sortedPos4", C.c_void_p),
                 ("sortedVel4", C.c_void_p), ("hash", C.c_void_p), ("index", C.c_void_p),
@@ -80,6 +92,13 @@ def _load() -> C.CDLL:
         "orc_run": (None, [C.POINTER(OrcState), f, u32, P]),
         "orc_reset": (None, [vp, vp, u32, C.c_int, C.c_uint64, f, f, f]),
         "orc_energy_momentum": (None, [vp, u32, vp]),
+        "orc_distmap_query_batch": (None, [C.POINTER(OrcDistMap), vp, u32, vp, vp, vp]),
+        "orc_integrate_distmap": (None, [vp, vp, u32, f, P, C.POINTER(OrcDistMap)]),
+        "orc_distmap_eval": (None, [C.POINTER(OrcDistMap), vp, u32, C.POINTER(OrcTransInfo), vp,
+                                    vp, vp]),
+        "orc_point_triangle_dist2": (C.c_double, [vp, vp, vp, vp, vp]),
+        "orc_mesh_sdf": (None, [vp, vp, u32, vp, u32, vp, vp, vp]),
+        "orc_update_distmap": (None, [C.POINTER(OrcState), f, P, C.POINTER(OrcDistMap)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -236,3 +255,102 @@ def energy_momentum(vel4) -> np.ndarray:
     out = np.zeros(4, np.float64)
     lib.orc_energy_momentum(_p(vel4), len(vel4), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- distance map
+class DistMap:
+    """Host arrays of a tensor-product distance map for the oracle."""
+
+    def __init__(self, cells, cell_size, bmin, bmax, xs, ys, zs, dist, cp, nrm):
+        self.arrays = [np.ascontiguousarray(a, np.float32) for a in (xs, ys, zs, dist, cp, nrm)]
+        self.s = OrcDistMap()
+        for a in range(3):
+            self.s.cells[a] = int(cells[a])
+            self.s.bmin[a] = float(bmin[a])
+            self.s.bmax[a] = float(bmax[a])
+        self.s.cellSize = float(cell_size)
+        self.s.xs, self.s.ys, self.s.zs, self.s.dist, self.s.cp, self.s.nrm = [
+            _p(a) for a in self.arrays]
+
+    @classmethod
+    def from_reference_layout(cls, cells, cell_size, bmin, bmax, coords, nrm, cp, dist):
+        cx, cy, cz = (int(c) for c in cells)
+        coords = np.asarray(coords, np.float32).reshape(-1, 3)
+        ix, isl = cx + 1, (cx + 1) * (cy + 1)
+        xs = coords[np.arange(cx + 1), 0]
+        ys = coords[np.arange(cy + 1) * ix, 1]
+        zs = coords[np.arange(cz + 1) * isl, 2]
+        return cls(cells, cell_size, bmin, bmax, xs, ys, zs, dist, np.asarray(cp).reshape(-1),
+                   np.asarray(nrm).reshape(-1))
+
+
+def distmap_query(m: DistMap, pts):
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    n = len(pts)
+    d = np.zeros(n, np.float32)
+    cp = np.zeros((n, 3), np.float32)
+    nr = np.zeros((n, 3), np.float32)
+    lib.orc_distmap_query_batch(C.byref(m.s), _p(pts), n, _p(d), _p(cp), _p(nr))
+    return d, cp, nr
+
+
+def integrate_distmap(pos4, vel4, dt, P, m: DistMap):
+    pos4 = np.array(pos4, np.float32, copy=True, order="C")
+    vel4 = np.array(vel4, np.float32, copy=True, order="C")
+    lib.orc_integrate_distmap(_p(pos4), _p(vel4), len(pos4), dt, C.byref(P), C.byref(m.s))
+    return pos4, vel4
+
+
+def distmap_eval(m: DistMap, v, info: OrcTransInfo, cps=None, normals=None):
+    v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+    n = len(v)
+    cps = np.zeros((n, 3), np.float32) if cps is None else np.array(cps, np.float32)
+    normals = np.zeros((n, 3), np.float32) if normals is None else np.array(normals, np.float32)
+    dist = np.zeros(n, np.float32)
+    lib.orc_distmap_eval(C.byref(m.s), _p(v), n, C.byref(info), _p(cps), _p(normals), _p(dist))
+    return cps, normals, dist
+
+
+def point_triangle_dist2(q, a, b, c):
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (q, a, b, c)]
+    cp = np.zeros(3)
+    d2 = lib.orc_point_triangle_dist2(*[_p(x) for x in arrs], _p(cp))
+    return d2, cp
+
+
+def mesh_sdf(verts, tris, pts):
+    verts = np.ascontiguousarray(verts, np.float64)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    sdf = np.zeros(n)
+    cp = np.zeros((n, 3))
+    w = np.zeros(n)
+    lib.orc_mesh_sdf(_p(verts), _p(tris), len(tris), _p(pts), n, _p(sdf), _p(cp), _p(w))
+    return sdf, cp, w
+
+
+def update_distmap(state: "State", dt, P, m: DistMap):
+    lib.orc_update_distmap(C.byref(state._s), dt, C.byref(P), C.byref(m.s))
+
+
+# ------------------------------------------------- the reference's own build
+REF_LIB = HERE / "_ref" / "libref_distance.so"
+
+
+def reference_point_triangle():
+    """FullC0ntact's CDistancePointTriangle<double>::ComputeDistance, built
+    from the reference sources by oracle/ref.mk; None when not built."""
+    if not REF_LIB.exists():
+        return None
+    L = C.CDLL(str(REF_LIB))
+    fn = L.ref_point_triangle_distance
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_void_p] * 5
+
+    def call(q, a, b, c):
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (q, a, b, c)]
+        cp = np.zeros(3)
+        d = fn(*[_p(x) for x in arrs], _p(cp))
+        return d, cp
+    return call
