@@ -6,9 +6,11 @@ shear .1, boundary damping -.5, gravity -0.0003, dt 0.5.  A step is one
 ParticleSystem::update: integrate -> hash -> sort -> reorder -> collide.
 
 Prints ONE JSON line (the last line of stdout) with the throughput in
-particle-steps/s, the HBM roofline of the collide kernel (algorithmic bytes
-over its HIP-event-timed average duration) and the CPU baseline (the oracle,
-the CPU restatement of the same step, timed on this host).
+particle-steps/s of the multi-step run (graph-launched), the HBM roofline of
+the dominant kernel of that path (algorithmic bytes over its HIP-event-timed
+average duration; HBM traffic from the committed rocprofv3 counter summary)
+and the CPU baseline (the oracle, the CPU restatement of the same step,
+timed on this host).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -67,7 +69,38 @@ def cell_counts(cs: np.ndarray, grid: int) -> tuple[int, int]:
     return int(len(occ)), int(mark.sum())
 
 
-def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
+def stage_bytes(name: str, n: int, c_occ: int, c_touch: int) -> int | None:
+    """Algorithmic HBM bytes of one launch of a stage: every array it must
+    touch read or written once, cell tables counted per occupied (or
+    neighbourhood-touched) cell, no dense table clears."""
+    if name == "collide":  # own pos/vel 32 B, index 4 B, newVel 16 B + table probes
+        return 52 * n + 8 * c_touch
+    if name == "collide_integrate":  # + integrate/hash epilogue: pos/vel 32 B, key/value 8 B
+        return 76 * n + 8 * c_touch  # (the newVel store and reload are fused away)
+    if name == "reorder":  # hash+index 8 B, gather 32 B, write 32 B, tables 8 B/cell
+        return 72 * n + 8 * c_occ
+    if name.startswith("sort_pass"):  # keys+values read and written once
+        return 16 * n
+    if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
+        return 72 * n
+    return None
+
+
+def profiled_traffic(kernel: str) -> int | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size)."""
+    f = ROOT / "profiles" / "traffic_dem-1M-spheres.md"
+    try:
+        for ln in f.read_text().splitlines():
+            parts = ln.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def gpu_leg(steps: int, warmup: int, timing_steps: int, progress_every: int = 0) -> dict:
     from fullc0ntact_amd import ParticleSystem, make_params
 
     ps = ParticleSystem(N, (GRID, GRID, GRID))
@@ -79,8 +112,15 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
     ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
     ps.sync()
     log(f"[bench] warmup {warmup} steps")
-    ps.run(DT, warmup)
-    ps.sync()
+    chunk = progress_every if progress_every > 0 else warmup
+    done = 0
+    while done < warmup:
+        k = min(chunk, warmup - done)
+        ps.run(DT, k)
+        ps.sync()
+        done += k
+        if progress_every > 0:
+            log(f"[bench] warmup step {done}")
     log(f"[bench] timed {steps} steps")
     t0 = time.perf_counter()
     ps.run(DT, steps)
@@ -88,28 +128,26 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int) -> dict:
     t1 = time.perf_counter()
     elapsed = t1 - t0
 
-    # Per-stage durations: HIP events around every launch on the system's
-    # stream, over a second timed region of the same workload.
-    log(f"[bench] per-stage event timing over {timing_steps} steps")
+    # Kernel durations: HIP events around every launch on the system's
+    # stream, over a further run of the same multi-step path (launched
+    # eagerly so that each launch is bracketed), then over single updates.
+    log(f"[bench] per-kernel event timing over {timing_steps} steps")
     ctx = ps.ctx
     ctx.enable_timing(True)
     ctx.reset_timing()
-    for _ in range(timing_steps):
-        ps.update(DT)
-    stages = ctx.timing()
-    # The multi-step run fuses each collide with the next update's integrate;
-    # time its kernels too (logged only).
-    ctx.reset_timing()
     ps.run(DT, timing_steps)
     run_stages = ctx.timing()
+    ctx.reset_timing()
+    for _ in range(timing_steps):
+        ps.update(DT)
+    update_stages = ctx.timing()
     ctx.enable_timing(False)
     ps.sync()
-    for k, v in run_stages.items():
-        log(f"[bench] run stage {k:18s} {v[1] / max(v[0], 1) * 1e3:10.2f} us x {v[0]}")
     cs = ps.download("cellStart")
     c_occ, c_touch = cell_counts(cs, GRID)
     ps.close()
-    return dict(elapsed=elapsed, stages=stages, c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs)
+    return dict(elapsed=elapsed, run_stages=run_stages, update_stages=update_stages,
+                c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs)
 
 
 def cpu_leg(target_seconds: float, threads: int) -> dict:
@@ -138,29 +176,40 @@ def main(argv=None) -> int:
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--progress-every", type=int, default=0,
+                    help="run the warmup in chunks of this many steps, logging each")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported "
               "(this build has no multi-GPU path).", file=sys.stderr)
         return 2
 
-    g = gpu_leg(a.steps, a.warmup, a.timing_steps)
+    g = gpu_leg(a.steps, a.warmup, a.timing_steps, a.progress_every)
     ms = g["elapsed"] / a.steps * 1e3
     value = N * a.steps / g["elapsed"]
-    st = g["stages"]
-    stage_us = {k: round(v[1] / max(v[0], 1) * 1e3, 3) for k, v in st.items()}
-    for k, v in stage_us.items():
-        log(f"[bench] stage {k:16s} {v:10.2f} us")
 
-    collide_us = stage_us.get("collide")
-    collide_bytes = 52 * N + 8 * g["c_touch"]
+    def per_launch_us(stages):
+        return {k: round(v[1] / max(v[0], 1) * 1e3, 3) for k, v in stages.items()}
+
+    run_us = per_launch_us(g["run_stages"])
+    upd_us = per_launch_us(g["update_stages"])
+    for k, v in g["run_stages"].items():
+        log(f"[bench] run    {k:18s} {run_us[k]:10.2f} us x {v[0]}")
+    for k, v in upd_us.items():
+        log(f"[bench] update {k:18s} {v:10.2f} us")
+
+    # The dominant kernel of the timed path: the largest total event time.
+    dom = max(g["run_stages"].items(), key=lambda kv: kv[1][1])[0]
+    nbytes = stage_bytes(dom, N, g["c_occ"], g["c_touch"])
     roof = None
-    if collide_us:
-        achieved = collide_bytes / (collide_us * 1e-6) / 1e9
+    if nbytes:
+        us = run_us[dom]
+        achieved = nbytes / (us * 1e-6) / 1e9
+        traffic = profiled_traffic(dom)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "collide", "bytes_per_launch": collide_bytes,
-                "avg_launch_us": collide_us, "copy_probe_GBs": round(g["copy_gbs"], 1)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": dom, "bytes_per_launch": nbytes, "avg_launch_us": us,
+                "copy_probe_GBs": round(g["copy_gbs"], 1)}
 
     cpu = None
     if not a.no_cpu:
@@ -191,7 +240,7 @@ def main(argv=None) -> int:
                    "C_touch": g["c_touch"]},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "stages_us": stage_us,
+        "stages_us": {"run": run_us, "update": upd_us},
     }
     print(json.dumps(line), flush=True)
     return 0
