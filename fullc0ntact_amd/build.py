@@ -5,7 +5,9 @@ Usage: python -m fullc0ntact_amd.build [--jobs N] [--verbose]
 Every .hip file under csrc/ is compiled with hipcc to an object file under
 fullc0ntact_amd/_build/, then linked into fullc0ntact_amd/libfc_dem.so.  The
 library is always rebuilt from source (the build is the "does it compile"
-check); objects are reused only within one build invocation.
+check); objects are reused only within one build invocation.  The C++ test
+driver of the host API (tests/cpp/fc_particles_driver.cpp) is built against
+the library afterwards.
 """
 from __future__ import annotations
 
@@ -82,6 +84,29 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
+DRIVER_SRC = ROOT / "tests" / "cpp" / "fc_particles_driver.cpp"
+DRIVER = DRIVER_SRC.with_suffix("")
+
+
+def build_driver(verbose: bool = False) -> Path | None:
+    """Host-only C++ program over include/fc_particle_system.hpp; links
+    libfc_dem.so with an rpath relative to its own location."""
+    if not DRIVER_SRC.exists():
+        return None
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("g++ not found: needed for the C++ host-API driver")
+    rel = os.path.relpath(PKG, DRIVER.parent)
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(DRIVER_SRC),
+           f"-L{PKG}", "-lfc_dem", f"-Wl,-rpath,$ORIGIN/{rel}", "-o", str(DRIVER)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    return DRIVER
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--jobs", type=int, default=None)
@@ -89,6 +114,9 @@ def main(argv: list[str] | None = None) -> int:
     a = ap.parse_args(argv)
     lib = build(a.jobs, a.verbose)
     print(f"built {lib}")
+    drv = build_driver(a.verbose)
+    if drv:
+        print(f"built {drv}")
     return 0
 
 
