@@ -152,6 +152,11 @@ _SIGS = {
     "fc_distmap_eval": (_st, [_vp, _vp, _vp, _u32, C.POINTER(TransInfo), _vp, _vp, _vp]),
     "fc_integrate_system_distmap": (_st, [_vp, _vp, _vp, _vp, C.c_float, _u32]),
     "fc_mesh_signed_distance": (_st, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp]),
+    "fc_system_set_polydisperse": (_st, [_vp, C.c_int]),
+    "fc_stats_fp64": (_st, [_vp, _vp, _u32, C.POINTER(C.c_double)]),
+    "fc_system_stats_fp64": (_st, [_vp, C.POINTER(C.c_double)]),
+    "fc_contact_histogram": (_st, [_vp, _vp, _vp, _vp, _u32, C.c_int, _vp, _u32]),
+    "fc_system_contact_histogram": (_st, [_vp, _vp, _u32]),
 }
 
 

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(fc_sim_params P, float4* _
   float4 p4 = pos[i];
   float4 v4 = vel[i];
   v3f p = xyz(p4), v = xyz(v4);
-  integrate_box(P, dt, p, v);
+  integrate_box(P, P.particleRadius, dt, p, v);
   pos[i] = make_float4(p.x, p.y, p.z, p4.w);
   vel[i] = make_float4(v.x, v.y, v.z, v4.w);
 }
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
     const float4 p4 = pos[i];
     const float4 v4 = vel[i];
     v3f p = xyz(p4), v = xyz(v4);
-    integrate_with(P, B, dt, p, v);
+    integrate_with(P, B, p4.w, dt, p, v);
     pos[i] = make_float4(p.x, p.y, p.z, p4.w);
     vel[i] = make_float4(v.x, v.y, v.z, v4.w);
     key = hash_of(P, p.x, p.y, p.z);
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // with a one-ahead prefetch of the next neighbour's position and velocity.
 __device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel, uint32_t k,
                                         uint32_t s, uint32_t e, const float4* __restrict__ sp,
-                                        const float4* __restrict__ sv, float r) {
+                                        const float4* __restrict__ sv, float rA, bool poly) {
   v3f cf = make_v3(0.0f, 0.0f, 0.0f);
   if (s >= e) return cf;
   float4 pn = sp[s], vn = sv[s];
@@ -159,8 +159,8 @@ __device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel
       vn = sv[j + 1];
     }
     if (j != k) {
-      v3f f = collide_spheres(pos, xyz(pc), vel, xyz(vc), r, r, P.spring, P.damping, P.shear,
-                              P.attraction);
+      v3f f = collide_spheres(pos, xyz(pc), vel, xyz(vc), rA, poly ? pc.w : rA, P.spring,
+                              P.damping, P.shear, P.attraction);
       cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
     }
   }
@@ -181,15 +181,18 @@ __device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel
 __global__ __launch_bounds__(kBlock) void k_collide_v2(
     fc_sim_params P, float4* __restrict__ newVel, const float4* __restrict__ sortedPos,
     const float4* __restrict__ sortedVel, const uint32_t* __restrict__ index,
-    const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n) {
+    const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n,
+    int poly, uint32_t* devFlags) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
-  const v3f pos = xyz(sortedPos[k]);
+  const float4 p4 = sortedPos[k];
+  const v3f pos = xyz(p4);
   const v3f vel = xyz(sortedVel[k]);
   const int gx = cell_coord(pos.x, P.worldOrigin.x, P.cellSize.x);
   const int gy = cell_coord(pos.y, P.worldOrigin.y, P.cellSize.y);
   const int gz = cell_coord(pos.z, P.worldOrigin.z, P.cellSize.z);
-  const float r = P.particleRadius;
+  const float r = poly ? p4.w : P.particleRadius;
+  if (poly && !(r <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
   v3f force = make_v3(0.0f, 0.0f, 0.0f);
 #pragma unroll 1
   for (int dz = -1; dz <= 1; ++dz) {
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
     for (int c = 0; c < 9; ++c) {
       if (st[c] != CELL_EMPTY) {
         const uint32_t e = en[c] > n ? n : en[c];
-        v3f cf = cell_sum(P, pos, vel, k, st[c], e, sortedPos, sortedVel, r);
+        v3f cf = cell_sum(P, pos, vel, k, st[c], e, sortedPos, sortedVel, r, poly != 0);
         force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
       }
     }
@@ -266,8 +269,12 @@ constexpr int kBatch = 4;
 //     so the result is bit-identical to the 27-cell loop, while the wave
 //     spends the expensive pair arithmetic only on (near-)contacts.
 // A lane whose list would overflow falls back to the direct 27-cell loop.
+// kPoly: radii come from pos.w (rA for this sphere); contactD2 is then the
+// threshold of the largest contact distance 2 * particleRadius, which
+// bounds every rA + rB.
+template <bool kPoly>
 __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float contactD2,
-                                              uint32_t k, v3f pos, v3f vel,
+                                              uint32_t k, v3f pos, v3f vel, float rA,
                                               const float4* __restrict__ sortedPos,
                                               const float4* __restrict__ sortedVel,
                                               const uint32_t* __restrict__ cellStart,
@@ -276,7 +283,6 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
   const int gx = cell_coord(pos.x, P.worldOrigin.x, P.cellSize.x);
   const int gy = cell_coord(pos.y, P.worldOrigin.y, P.cellSize.y);
   const int gz = cell_coord(pos.z, P.worldOrigin.z, P.cellSize.z);
-  const float r = P.particleRadius;
 
   // ---- phase A ----
   // All 27 table entries are requested before any is used (one latency).
@@ -356,8 +362,8 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
             cf = make_v3(0.0f, 0.0f, 0.0f);
             curCell = cell;
           }
-          v3f f = collide_spheres(pos, xyz(qp[u]), vel, xyz(qv[u]), r, r, P.spring, P.damping,
-                                  P.shear, P.attraction);
+          v3f f = collide_spheres(pos, xyz(qp[u]), vel, xyz(qv[u]), rA, kPoly ? qp[u].w : rA,
+                                  P.spring, P.damping, P.shear, P.attraction);
           cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
         }
       }
@@ -374,15 +380,17 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
           v3f cf = make_v3(0.0f, 0.0f, 0.0f);
           for (uint32_t j = start; j < end; ++j)
             if (j != k) {
-              v3f f = collide_spheres(pos, xyz(sortedPos[j]), vel, xyz(sortedVel[j]), r, r,
-                                      P.spring, P.damping, P.shear, P.attraction);
+              const float4 pj = sortedPos[j];
+              v3f f = collide_spheres(pos, xyz(pj), vel, xyz(sortedVel[j]), rA,
+                                      kPoly ? pj.w : rA, P.spring, P.damping, P.shear,
+                                      P.attraction);
               cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
             }
           force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
         }
   }
   v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
-  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), r, P.colliderRadius,
+  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), rA, P.colliderRadius,
                           P.spring, P.damping, P.shear, 0.0f);
   force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
   return make_v3(vel.x + force.x, vel.y + force.y, vel.z + force.z);
@@ -395,13 +403,17 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
 // pos/newVel, the cell hash into keysOut, and the radix sort's digit
 // histogram of those keys.  Every value written is the one the separate
 // stages would write.
-template <bool kFuse>
+// kPoly takes each sphere's radius from pos.w (a radius above
+// particleRadius raises DEVERR_RADIUS: the neighbour search and the contact
+// pre-filter are sized by particleRadius).
+template <bool kFuse, bool kPoly>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
     fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
     const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
     const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
-    float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters) {
+    float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
+    uint32_t* devFlags) {
   __shared__ uint32_t s_list[kListCap][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -413,14 +425,16 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
   uint32_t key = 0;
   if (valid) {
     const float4 p4 = sortedPos[k];
-    const v3f nv = collide_one_v3(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), sortedPos,
-                                  sortedVel, cellStart, cellEnd, n, s_list);
+    const float rA = kPoly ? p4.w : P.particleRadius;
+    if (kPoly && !(rA <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
+    const v3f nv = collide_one_v3<kPoly>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
+                                         sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
     const uint32_t orig = index[k];
     if (!kFuse) {
       newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
     } else {
       v3f p = xyz(p4), v = nv;
-      integrate_with(P, B, dt, p, v);
+      integrate_with(P, B, p4.w, dt, p, v);
       pos[orig] = make_float4(p.x, p.y, p.z, p4.w);
       newVel[orig] = make_float4(v.x, v.y, v.z, 0.0f);
       key = hash_of(P, p.x, p.y, p.z);
@@ -507,22 +521,26 @@ float contact_d2_threshold(float R) {
   return t;
 }
 
-hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
-                          const float4* sortedPos, const float4* sortedVel,
+hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
+                          float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
+  const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   // The contact-list kernel packs a sorted index into 27 bits and assumes
   // the three cells of a neighbour row are distinct (G.x >= 4).
   if (n >= (1u << 27) || P.gridSize.x < 4)
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
-                       cellStart, cellEnd, n);
-  else
-    hipLaunchKernelGGL(k_collide_v3<false>, g, b, 0, s, P, BoundaryArg{},
-                       contact_d2_threshold(P.particleRadius + P.particleRadius), newVel,
+                       cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
+  else if (poly)
+    hipLaunchKernelGGL((k_collide_v3<false, true>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
-                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr);
+                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr, devFlags);
+  else
+    hipLaunchKernelGGL((k_collide_v3<false, false>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
+                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
+                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr, devFlags);
   return hipGetLastError();
 }
 
@@ -537,10 +555,16 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_collide_v3<true>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, B,
-                     contact_d2_threshold(P.particleRadius + P.particleRadius), vel, sortedPos,
-                     sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut,
-                     passes, sort.histogram, sort.counters);
+  const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
+  const dim3 g(grid_for(n, kBlock)), b(kBlock);
+  if (B.poly)
+    hipLaunchKernelGGL((k_collide_v3<true, true>), g, b, 0, s, P, B, d2, vel, sortedPos, sortedVel,
+                       index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut, passes,
+                       sort.histogram, sort.counters, devFlags);
+  else
+    hipLaunchKernelGGL((k_collide_v3<true, false>), g, b, 0, s, P, B, d2, vel, sortedPos,
+                       sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt,
+                       keysOut, passes, sort.histogram, sort.counters, devFlags);
   return hipGetLastError();
 }
 

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_distmap(fc_sim_params P, D
   if (i >= n) return;
   const float4 p4 = pos[i], v4 = vel[i];
   v3f p = xyz(p4), v = xyz(v4);
-  integrate_distmap(P, m, dt, p, v);
+  integrate_distmap(P, m, P.particleRadius, dt, p, v);
   pos[i] = make_float4(p.x, p.y, p.z, p4.w);
   vel[i] = make_float4(v.x, v.y, v.z, v4.w);
 }

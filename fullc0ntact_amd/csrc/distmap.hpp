@@ -93,13 +93,12 @@ __device__ __forceinline__ v3f normalize_ref(v3f v) {
 // d_integrateSystem (particledem_um.cu:604-638) with d_checkBoundaryDistMap
 // and checkDistMap (boundaries.cuh:38-89): semi-implicit Euler, push-out of
 // the distance-map boundary, a cylinder of radius 1 around the z axis, and
-// the planes z = -1 and z = 10.
+// the planes z = -1 and z = 10.  r is the sphere's radius.
 __device__ __forceinline__ void integrate_distmap(const fc_sim_params& P, const DistMapView& m,
-                                                  float dt, v3f& pos, v3f& vel) {
+                                                  float r, float dt, v3f& pos, v3f& vel) {
   vel = make_v3(vel.x + P.gravity.x * dt, vel.y + P.gravity.y * dt, vel.z + P.gravity.z * dt);
   vel = make_v3(vel.x * P.globalDamping, vel.y * P.globalDamping, vel.z * P.globalDamping);
   pos = make_v3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
-  const float r = P.particleRadius;
   const float bd = P.boundaryDamping;
   const v3f pc = make_v3(pos.x, pos.y, 0.0f);  // taken before the push-out
   const float len = sqrtf(dot3(pc, pc));
@@ -129,18 +128,21 @@ __device__ __forceinline__ void integrate_distmap(const fc_sim_params& P, const 
 }
 
 // Boundary of a particle system: the box walls of integrate_functor, or
-// the distance-map boundary when hasMap.
+// the distance-map boundary when hasMap; poly takes each sphere's radius
+// from pos.w instead of the parameter radius.
 struct BoundaryArg {
   int hasMap;
+  int poly;
   DistMapView map;
 };
 
 __device__ __forceinline__ void integrate_with(const fc_sim_params& P, const BoundaryArg& B,
-                                               float dt, v3f& pos, v3f& vel) {
+                                               float w, float dt, v3f& pos, v3f& vel) {
+  const float r = B.poly ? w : P.particleRadius;
   if (B.hasMap)
-    integrate_distmap(P, B.map, dt, pos, vel);
+    integrate_distmap(P, B.map, r, dt, pos, vel);
   else
-    integrate_box(P, dt, pos, vel);
+    integrate_box(P, r, dt, pos, vel);
 }
 
 }  // namespace fc

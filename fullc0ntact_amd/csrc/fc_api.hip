@@ -190,6 +190,9 @@ fc_status check_device_flags(fc_context* ctx) {
     return set_error(FC_ERROR_SORT_STALLED, "radix sort look-back exceeded its spin bound");
   if (flags & DEVERR_HIST_STALLED)
     return set_error(FC_ERROR_SORT_STALLED, "radix sort histogram never completed");
+  if (flags & DEVERR_RADIUS)
+    return set_error(FC_ERROR_INVALID_VALUE,
+                     "a sphere radius (pos.w) is not <= particleRadius in polydisperse mode");
   return set_error(FC_ERROR_INVALID_VALUE,
                    "a sort key had bits above log2(numCells): hashes must be < numCells");
 }
@@ -259,6 +262,7 @@ fc_status fc_context_destroy(fc_context* ctx) {
   hipFree(s.histogram);
   hipFree(s.counters);
   hipFree(ctx->devFlags);
+  hipFree(ctx->statsScratch);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return FC_SUCCESS;
@@ -380,7 +384,7 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
     return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
   if (numCells != ctx->params.numCells)
     return set_error(FC_ERROR_INVALID_VALUE, "numCells differs from the context parameters");
-  hipError_t e = launch_collide(ctx->stream, ctx->params, (float4*)newVel,
+  hipError_t e = launch_collide(ctx->stream, ctx->params, false, ctx->devFlags, (float4*)newVel,
                                 (const float4*)sortedPos, (const float4*)sortedVel, index,
                                 cellStart, cellEnd, numParticles);
   if (e != hipSuccess) return hip_error(e, "collide");

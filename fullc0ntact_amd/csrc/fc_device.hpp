@@ -90,12 +90,13 @@ __device__ __forceinline__ v3f collide_spheres(v3f pA, v3f pB, v3f vA, v3f vB, f
 
 // integrate_functor body (particles_kernel.cu:46-74): semi-implicit Euler
 // and the box walls at +-1 (not derived from worldOrigin), tested in the
-// order x+, x-, y+, z+, z-, y-.
-__device__ __forceinline__ void integrate_box(const fc_sim_params& P, float dt, v3f& pos, v3f& vel) {
+// order x+, x-, y+, z+, z-, y-.  r is the sphere's radius (the parameter
+// radius, or pos.w for polydisperse systems).
+__device__ __forceinline__ void integrate_box(const fc_sim_params& P, float r, float dt, v3f& pos,
+                                              v3f& vel) {
   vel = make_v3(vel.x + P.gravity.x * dt, vel.y + P.gravity.y * dt, vel.z + P.gravity.z * dt);
   vel = make_v3(vel.x * P.globalDamping, vel.y * P.globalDamping, vel.z * P.globalDamping);
   pos = make_v3(pos.x + vel.x * dt, pos.y + vel.y * dt, pos.z + vel.z * dt);
-  const float r = P.particleRadius;
   const float hi = 1.0f - r;
   const float lo = -1.0f + r;
   const float bd = P.boundaryDamping;

@@ -39,6 +39,7 @@ enum : uint32_t {
   DEVERR_SORT_STALLED = 1u << 0,   // look-back exceeded its spin bound
   DEVERR_SORT_KEY_RANGE = 1u << 1, // a key had bits above the sorted width
   DEVERR_HIST_STALLED = 1u << 2,   // histogram totals never reached n
+  DEVERR_RADIUS = 1u << 3,         // polydisperse radius above particleRadius
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
@@ -152,8 +153,9 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
                           uint32_t numCells);
-hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, float4* newVel,
-                          const float4* sortedPos, const float4* sortedVel,
+// poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
+hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
+                          float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n);
 
@@ -188,6 +190,7 @@ struct fc_context {
   fc::SortScratch sort;
   uint32_t* devFlags = nullptr;   // device error word
   fc::StageTimer timer;
+  double* statsScratch = nullptr; // FP64 reduction partials + histogram bins
 };
 
 struct fc_system {
@@ -214,5 +217,7 @@ struct fc_system {
   // Boundary of the integrate stage.
   fc_boundary boundary = FC_BOUNDARY_BOX;
   const fc_distmap* boundaryMap = nullptr;
+  // Radii from pos.w (set by fc_system_set_polydisperse).
+  int polydisperse = 0;
   fc::BoundaryArg boundary_arg() const;
 };

@@ -104,8 +104,8 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "reorder");
   if (timer) timer->begin(st, "collide");
-  e = launch_collide(st, P, sys->vel, sys->sortedPos, sys->sortedVel, sys->index, sys->cellStart,
-                     sys->cellEnd, n);
+  e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
+                     sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   sys->tablesTrackHash = true;
@@ -172,8 +172,8 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   if (e != hipSuccess) return hip_error(e, "reorder");
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
   if (last)
-    e = launch_collide(st, P, sys->vel, sys->sortedPos, sys->sortedVel, sys->index,
-                       sys->cellStart, sys->cellEnd, n);
+    e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
+                       sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n);
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel,
@@ -216,6 +216,7 @@ using namespace fc;
 fc::BoundaryArg fc_system::boundary_arg() const {
   fc::BoundaryArg b{};
   b.hasMap = (boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr) ? 1 : 0;
+  b.poly = polydisperse ? 1 : 0;
   if (b.hasMap) b.map = distmap_view(boundaryMap);
   return b;
 }
@@ -232,6 +233,14 @@ fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_dist
   sys->boundary = mode;
   sys->boundaryMap = mode == FC_BOUNDARY_DISTMAP ? map : nullptr;
   drop_graph(sys);  // the captured step carries the boundary in its arguments
+  return FC_SUCCESS;
+}
+
+fc_status fc_system_set_polydisperse(fc_system* sys, int enable) {
+  if (!sys) return set_error(FC_ERROR_INVALID_VALUE, "sys is NULL");
+  FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  sys->polydisperse = enable ? 1 : 0;
+  drop_graph(sys);  // the captured step carries the radius mode in its kernels
   return FC_SUCCESS;
 }
 

@@ -83,6 +83,25 @@ class ParticleSystem:
     def sync(self) -> None:
         self.ctx.sync()
 
+    def setPolydisperse(self, enable: bool = True) -> None:
+        """Radii from pos.w (m_params.particleRadius must be the largest
+        radius and the cell size at least twice it).  No counterpart in the
+        reference, which writes pos.w = 1 and ignores it."""
+        check(lib.fc_system_set_polydisperse(self.handle, 1 if enable else 0))
+
+    # -- observables (FP64 reductions on the GPU) -----------------------------
+    def stats_fp64(self) -> tuple[float, np.ndarray]:
+        """Kinetic energy 1/2 sum |v|^2 and momentum sum v (unit masses)."""
+        out = (C.c_double * 4)()
+        check(lib.fc_system_stats_fp64(self.handle, out))
+        return float(out[0]), np.array(out[1:4], np.float64)
+
+    def contactHistogram(self, nbins: int = 32) -> np.ndarray:
+        """Spheres per contact count at the last collide (last bin: >=)."""
+        h = np.zeros(nbins, np.uint32)
+        check(lib.fc_system_contact_histogram(self.handle, h.ctypes.data, int(nbins)))
+        return h
+
     def setBoundary(self, distance_map=None) -> None:
         """Integrate against the distance-map boundary of
         d_integrateSystem/d_checkBoundaryDistMap (particledem_um.cu:604-638,

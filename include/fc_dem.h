@@ -228,6 +228,18 @@ fc_status fc_system_dump_grid(fc_system* sys, uint32_t* maxParticlesPerCell, uin
  * outlive its use by the system. */
 fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_distmap* map);
 
+/* Polydisperse spheres.  No counterpart in the reference, which writes
+ * pos.w = 1 and ignores it (particleSystem.cpp:392,422).  When enabled, pos.w
+ * holds each sphere's radius: the collide uses rA + rB per pair
+ * (collideSpheres, particles_kernel.cu:185-218, with per-sphere radii) and
+ * the walls and the distance-map boundary use the sphere's own radius.
+ * the parameters' particleRadius must be the largest radius (it sizes the
+ * neighbour search and the contact pre-filter; the cell size must be at
+ * least twice it); a larger pos.w is reported as FC_ERROR_INVALID_VALUE at
+ * the next synchronising call.  With every pos.w equal to particleRadius
+ * the results are bit-identical to the monodisperse step. */
+fc_status fc_system_set_polydisperse(fc_system* sys, int enable);
+
 /* ------------------------------------------------------------------------- */
 /* Distance maps (DistanceMap<float,gpu>, cuda_addon/distancemap.cuh:6-303;  */
 /* difi.cu).                                                                  */
@@ -282,6 +294,30 @@ fc_status fc_mesh_signed_distance(fc_context* ctx, const double* vertices, uint3
                                   const uint32_t* triangles, uint32_t numTriangles,
                                   const double* points, uint32_t numPoints, double* sdf,
                                   double* closest);
+
+/* ------------------------------------------------------------------------- */
+/* Global observables.  No counterpart in the reference.  FP64 is used only   */
+/* here, in the reductions; the particle state stays FP32.                   */
+/* ------------------------------------------------------------------------- */
+
+/* Kinetic energy 1/2 sum |v|^2 (unit mass) and momentum sum v of numParticles
+ * float4 velocities (device array), accumulated in binary64 by a fixed-order
+ * two-level reduction (deterministic for a given n).  out: host double[4] =
+ * {energy, px, py, pz}; synchronises the context stream. */
+fc_status fc_stats_fp64(fc_context* ctx, const float* vel, uint32_t numParticles, double out[4]);
+/* The same for a system's current velocities. */
+fc_status fc_system_stats_fp64(fc_system* sys, double out[4]);
+/* Contact-count histogram of a sorted state (device arrays as produced by
+ * fc_reorder_data_and_find_cell_start): for every sphere, the number of
+ * neighbours in its 27 cells with sqrtf(d.d) < rA + rB (the collide's test);
+ * hist[c] (device, nbins <= 4096 counters) receives the number of spheres
+ * with c contacts, the last bin collecting c >= nbins-1.  Radii are the
+ * context's particleRadius, or pos.w when polydisperse != 0. */
+fc_status fc_contact_histogram(fc_context* ctx, const float* sortedPos, const uint32_t* cellStart,
+                               const uint32_t* cellEnd, uint32_t numParticles, int polydisperse,
+                               uint32_t* hist, uint32_t nbins);
+/* The same for a system's state at its last collide; hist is a host array. */
+fc_status fc_system_contact_histogram(fc_system* sys, uint32_t* hist, uint32_t nbins);
 
 /* ------------------------------------------------------------------------- */
 /* Per-stage timing with HIP events on the context stream.                   */

@@ -248,6 +248,17 @@ class ParticleSystem {
                                  map ? map->get() : nullptr));
   }
 
+  // Radii from pos.w (m_params.particleRadius is then the largest radius).
+  void setPolydisperse(bool enable) { check(fc_system_set_polydisperse(sys_, enable ? 1 : 0)); }
+  // {kinetic energy, px, py, pz} of the current velocities, FP64 on the GPU.
+  void statsFp64(double out[4]) { check(fc_system_stats_fp64(sys_, out)); }
+  // Spheres per contact count at the last collide (the last bin collects >=).
+  std::vector<uint32_t> contactHistogram(uint32_t nbins) {
+    std::vector<uint32_t> h(nbins, 0u);
+    check(fc_system_contact_histogram(sys_, h.data(), nbins));
+    return h;
+  }
+
   void setDamping(float x) { m_params.globalDamping = x; }
   void setGravity(float x) { m_params.gravity = fc_float3{0.0f, 0.0f, x}; }
   void setCollideSpring(float x) { m_params.spring = x; }

@@ -70,14 +70,17 @@ uint32_t orc_hash_point(float x, float y, float z, const orc_params* P) {
 }
 
 /* integrate_functor (particles_kernel.cu:46-74). */
-void orc_integrate(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P) {
-  const float r = P->particleRadius;
+/* integrate_functor (particles_kernel.cu:39-75); with poly the wall
+ * distance is each sphere's radius pos.w. */
+static void integrate_impl(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P,
+                           int poly) {
   const float bd = P->boundaryDamping;
   long i;
 #pragma omp parallel for schedule(static)
   for (i = 0; i < (long)n; ++i) {
     float* p = pos4 + 4 * i;
     float* v = vel4 + 4 * i;
+    const float r = poly ? p[3] : P->particleRadius;
     f3 pos = f3_load(p), vel = f3_load(v);
     vel = f3_add(vel, f3_scale(f3_make(P->gravity.x, P->gravity.y, P->gravity.z), dt));
     vel = f3_scale(vel, P->globalDamping);
@@ -91,6 +94,14 @@ void orc_integrate(float* pos4, float* vel4, uint32_t n, float dt, const orc_par
     p[0] = pos.x; p[1] = pos.y; p[2] = pos.z;
     v[0] = vel.x; v[1] = vel.y; v[2] = vel.z;
   }
+}
+
+void orc_integrate(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P) {
+  integrate_impl(pos4, vel4, n, dt, P, 0);
+}
+
+void orc_integrate_poly(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P) {
+  integrate_impl(pos4, vel4, n, dt, P, 1);
 }
 
 /* calcHashD (particles_kernel.cu:97-115). */
@@ -214,16 +225,17 @@ void orc_pair_force(const float* pA, const float* pB, const float* vA, const flo
   out3[0] = f.x; out3[1] = f.y; out3[2] = f.z;
 }
 
-/* collideD + collideCell (particles_kernel.cu:223-293). */
-void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel4,
-                 const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
-                 uint32_t n, const orc_params* P) {
-  const float r = P->particleRadius;
+/* collideD + collideCell (particles_kernel.cu:223-293); with poly each
+ * pair uses the spheres' own radii pos.w (the collider pair uses rA). */
+static void collide_impl(float* newVel4, const float* sortedPos4, const float* sortedVel4,
+                         const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
+                         uint32_t n, const orc_params* P, int poly) {
   long k;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (k = 0; k < (long)n; ++k) {
     f3 pos = f3_load(sortedPos4 + 4 * k);
     f3 vel = f3_load(sortedVel4 + 4 * k);
+    const float rA = poly ? sortedPos4[4 * k + 3] : P->particleRadius;
     int gx = grid_coord(pos.x, P->worldOrigin.x, P->cellSize.x);
     int gy = grid_coord(pos.y, P->worldOrigin.y, P->cellSize.y);
     int gz = grid_coord(pos.z, P->worldOrigin.z, P->cellSize.z);
@@ -238,9 +250,11 @@ void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel
             uint32_t end = cellEnd[h];
             for (uint32_t j = start; j < end; ++j) {
               if (j != (uint32_t)k) {
+                const float* pj = sortedPos4 + 4 * (size_t)j;
+                const float rB = poly ? pj[3] : P->particleRadius;
                 cellForce = f3_add(cellForce,
-                                   pair_force(pos, f3_load(sortedPos4 + 4 * (size_t)j), vel,
-                                              f3_load(sortedVel4 + 4 * (size_t)j), r, r,
+                                   pair_force(pos, f3_load(pj), vel,
+                                              f3_load(sortedVel4 + 4 * (size_t)j), rA, rB,
                                               P->attraction, P));
               }
             }
@@ -249,7 +263,7 @@ void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel
         }
     force = f3_add(force, pair_force(pos, f3_make(P->colliderPos.x, P->colliderPos.y,
                                                   P->colliderPos.z),
-                                     vel, f3_make(0.0f, 0.0f, 0.0f), r, P->colliderRadius, 0.0f,
+                                     vel, f3_make(0.0f, 0.0f, 0.0f), rA, P->colliderRadius, 0.0f,
                                      P));
     uint32_t orig = index[k];
     float* out = newVel4 + 4 * (size_t)orig;
@@ -257,6 +271,49 @@ void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel
     out[1] = vel.y + force.y;
     out[2] = vel.z + force.z;
     out[3] = 0.0f;
+  }
+}
+
+void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel4,
+                 const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
+                 uint32_t n, const orc_params* P) {
+  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 0);
+}
+
+void orc_collide_poly(float* newVel4, const float* sortedPos4, const float* sortedVel4,
+                      const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
+                      uint32_t n, const orc_params* P) {
+  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 1);
+}
+
+/* Contacts per sorted sphere: neighbours j != k in the 27 cells with
+ * sqrtf(d.d) < rA + rB, the collide's contact test. */
+void orc_contact_counts(uint32_t* counts, const float* sortedPos4, const uint32_t* cellStart,
+                        const uint32_t* cellEnd, uint32_t n, const orc_params* P, int poly) {
+  long k;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (k = 0; k < (long)n; ++k) {
+    f3 pos = f3_load(sortedPos4 + 4 * k);
+    const float rA = poly ? sortedPos4[4 * k + 3] : P->particleRadius;
+    int gx = grid_coord(pos.x, P->worldOrigin.x, P->cellSize.x);
+    int gy = grid_coord(pos.y, P->worldOrigin.y, P->cellSize.y);
+    int gz = grid_coord(pos.z, P->worldOrigin.z, P->cellSize.z);
+    uint32_t c = 0;
+    for (int z = -1; z <= 1; ++z)
+      for (int y = -1; y <= 1; ++y)
+        for (int x = -1; x <= 1; ++x) {
+          uint32_t h = grid_hash(gx + x, gy + y, gz + z, P);
+          uint32_t start = cellStart[h];
+          if (start == ORC_EMPTY) continue;
+          for (uint32_t j = start; j < cellEnd[h]; ++j) {
+            if (j == (uint32_t)k) continue;
+            const float* pj = sortedPos4 + 4 * (size_t)j;
+            const float rB = poly ? pj[3] : P->particleRadius;
+            f3 rel = f3_sub(f3_load(pj), pos);
+            if (sqrtf(f3_dot(rel, rel)) < rA + rB) ++c;
+          }
+        }
+    counts[k] = c;
   }
 }
 
@@ -343,6 +400,22 @@ void orc_update(orc_state* s, float dt, const orc_params* P) {
 
 void orc_run(orc_state* s, float dt, uint32_t steps, const orc_params* P) {
   for (uint32_t i = 0; i < steps; ++i) orc_update(s, dt, P);
+}
+
+/* One update with the radius mode and boundary of a particle system: poly
+ * takes radii from pos.w; a non-NULL map selects the distance-map boundary
+ * (d_integrateSystem, particledem_um.cu:604-638). */
+void orc_update_ex(orc_state* s, float dt, const orc_params* P, const orc_distmap* m, int poly) {
+  if (m)
+    orc_integrate_distmap_ex(s->pos4, s->vel4, s->n, dt, P, m, poly);
+  else
+    integrate_impl(s->pos4, s->vel4, s->n, dt, P, poly);
+  orc_calc_hash(s->hash, s->index, s->pos4, s->n, P);
+  orc_sort_pairs(s->hash, s->index, s->n);
+  orc_reorder(s->cellStart, s->cellEnd, s->sortedPos4, s->sortedVel4, s->hash, s->index,
+              s->pos4, s->vel4, s->n, P->numCells);
+  collide_impl(s->vel4, s->sortedPos4, s->sortedVel4, s->index, s->cellStart, s->cellEnd, s->n, P,
+               poly);
 }
 
 /* SplitMix64 (Steele, Lea, Flood 2014). */

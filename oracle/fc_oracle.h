@@ -93,6 +93,16 @@ typedef struct orc_state {
 void orc_update(orc_state* s, float dt, const orc_params* P);
 void orc_run(orc_state* s, float dt, uint32_t steps, const orc_params* P);
 
+/* Polydisperse variants (radius of each sphere in pos.w; the reference has
+ * none: with equal radii they reduce to orc_integrate / orc_collide). */
+void orc_integrate_poly(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P);
+void orc_collide_poly(float* newVel4, const float* sortedPos4, const float* sortedVel4,
+                      const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
+                      uint32_t n, const orc_params* P);
+/* Contacts per sorted sphere (the collide's test sqrtf(d.d) < rA + rB). */
+void orc_contact_counts(uint32_t* counts, const float* sortedPos4, const uint32_t* cellStart,
+                        const uint32_t* cellEnd, uint32_t n, const orc_params* P, int poly);
+
 /* reset(CONFIG_GRID / CONFIG_RANDOM) with SplitMix64 (see fc_dem.h). */
 void orc_reset(float* pos4, float* vel4, uint32_t n, int config, uint64_t seed, float radius,
                float jitterFrac, float velAmp);
@@ -141,6 +151,11 @@ double orc_point_triangle_dist2(const double q[3], const double B[3], const doub
 void orc_mesh_sdf(const double* verts, const uint32_t* tris, uint32_t nt, const double* pts,
                   uint32_t np, double* sdf, double* cp, double* winding);
 void orc_update_distmap(orc_state* s, float dt, const orc_params* P, const orc_distmap* m);
+void orc_integrate_distmap_ex(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P,
+                              const orc_distmap* m, int poly);
+/* One update with a radius mode (poly) and an optional distance-map
+ * boundary (m may be NULL: box walls). */
+void orc_update_ex(orc_state* s, float dt, const orc_params* P, const orc_distmap* m, int poly);
 
 #ifdef __cplusplus
 }

@@ -105,13 +105,19 @@ static inline void normalize3(const float v[3], float out[3]) {
  * particledem_um.cu:636). */
 void orc_integrate_distmap(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P,
                            const orc_distmap* m) {
-  const float r = P->particleRadius;
+  orc_integrate_distmap_ex(pos4, vel4, n, dt, P, m, 0);
+}
+
+/* With poly each sphere's radius is pos.w. */
+void orc_integrate_distmap_ex(float* pos4, float* vel4, uint32_t n, float dt, const orc_params* P,
+                              const orc_distmap* m, int poly) {
   const float bd = P->boundaryDamping;
   long i;
 #pragma omp parallel for schedule(static)
   for (i = 0; i < (long)n; ++i) {
     float* pp = pos4 + 4 * i;
     float* vv = vel4 + 4 * i;
+    const float r = poly ? pp[3] : P->particleRadius;
     float pos[3] = {pp[0], pp[1], pp[2]}, vel[3] = {vv[0], vv[1], vv[2]};
     const float g[3] = {P->gravity.x, P->gravity.y, P->gravity.z};
     for (int a = 0; a < 3; ++a) vel[a] = vel[a] + g[a] * dt;

@@ -99,6 +99,11 @@ def _load() -> C.CDLL:
         "orc_point_triangle_dist2": (C.c_double, [vp, vp, vp, vp, vp]),
         "orc_mesh_sdf": (None, [vp, vp, u32, vp, u32, vp, vp, vp]),
         "orc_update_distmap": (None, [C.POINTER(OrcState), f, P, C.POINTER(OrcDistMap)]),
+        "orc_integrate_poly": (None, [vp, vp, u32, f, P]),
+        "orc_collide_poly": (None, [vp, vp, vp, vp, vp, vp, u32, P]),
+        "orc_contact_counts": (None, [vp, vp, vp, vp, u32, P, C.c_int]),
+        "orc_integrate_distmap_ex": (None, [vp, vp, u32, f, P, C.POINTER(OrcDistMap), C.c_int]),
+        "orc_update_ex": (None, [C.POINTER(OrcState), f, P, C.POINTER(OrcDistMap), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -198,6 +203,39 @@ def collide(sp, sv, index, cs, ce, P, newVel4=None):
     lib.orc_collide(_p(out), _p(sp), _p(sv), _p(index), _p(np.ascontiguousarray(cs)),
                     _p(np.ascontiguousarray(ce)), n, C.byref(P))
     return out
+
+
+def integrate_poly(pos4, vel4, dt, P):
+    pos4 = np.array(pos4, np.float32, copy=True, order="C")
+    vel4 = np.array(vel4, np.float32, copy=True, order="C")
+    lib.orc_integrate_poly(_p(pos4), _p(vel4), len(pos4), dt, C.byref(P))
+    return pos4, vel4
+
+
+def collide_poly(sp, sv, index, cs, ce, P, newVel4=None):
+    n = len(sp)
+    out = np.zeros((n, 4), np.float32) if newVel4 is None else np.array(newVel4, np.float32)
+    sp = np.ascontiguousarray(sp, np.float32)
+    sv = np.ascontiguousarray(sv, np.float32)
+    index = np.ascontiguousarray(index, np.uint32)
+    lib.orc_collide_poly(_p(out), _p(sp), _p(sv), _p(index), _p(np.ascontiguousarray(cs)),
+                         _p(np.ascontiguousarray(ce)), n, C.byref(P))
+    return out
+
+
+def contact_counts(sp, cs, ce, P, poly=False):
+    """Contacts per sorted sphere (the collide's contact test)."""
+    sp = np.ascontiguousarray(sp, np.float32)
+    out = np.zeros(len(sp), np.uint32)
+    lib.orc_contact_counts(_p(out), _p(sp), _p(np.ascontiguousarray(cs, np.uint32)),
+                           _p(np.ascontiguousarray(ce, np.uint32)), len(sp), C.byref(P),
+                           1 if poly else 0)
+    return out
+
+
+def contact_histogram(sp, cs, ce, P, nbins, poly=False):
+    c = contact_counts(sp, cs, ce, P, poly)
+    return np.bincount(np.minimum(c, nbins - 1), minlength=nbins).astype(np.uint32)
 
 
 def collide_f64(sp, sv, index, cs, ce, P):
@@ -332,6 +370,19 @@ def mesh_sdf(verts, tris, pts):
 
 def update_distmap(state: "State", dt, P, m: DistMap):
     lib.orc_update_distmap(C.byref(state._s), dt, C.byref(P), C.byref(m.s))
+
+
+def update_ex(state: "State", dt, P, m: "DistMap | None" = None, poly: bool = False):
+    """One update with a radius mode and an optional distance-map boundary."""
+    lib.orc_update_ex(C.byref(state._s), dt, C.byref(P), C.byref(m.s) if m is not None else None,
+                      1 if poly else 0)
+
+
+def integrate_distmap_poly(pos4, vel4, dt, P, m: DistMap):
+    pos4 = np.array(pos4, np.float32, copy=True, order="C")
+    vel4 = np.array(vel4, np.float32, copy=True, order="C")
+    lib.orc_integrate_distmap_ex(_p(pos4), _p(vel4), len(pos4), dt, C.byref(P), C.byref(m.s), 1)
+    return pos4, vel4
 
 
 # ------------------------------------------------- the reference's own build
