@@ -76,6 +76,11 @@ class TransInfo(C.Structure):
                 ("w2m0", C.c_float * 9), ("m2w1", C.c_float * 9)]
 
 
+class BatchTuning(C.Structure):
+    """fc_batch_tuning: launch shape of the tuned batch kernels."""
+    _fields_ = [("integratePerThread", C.c_int), ("collideBlock", C.c_int), ("stepMs", C.c_float)]
+
+
 FC_BOUNDARY_BOX = 0
 FC_BOUNDARY_DISTMAP = 1
 
@@ -157,6 +162,22 @@ _SIGS = {
     "fc_system_stats_fp64": (_st, [_vp, C.POINTER(C.c_double)]),
     "fc_contact_histogram": (_st, [_vp, _vp, _vp, _vp, _u32, C.c_int, _vp, _u32]),
     "fc_system_contact_histogram": (_st, [_vp, _vp, _u32]),
+    "fc_batch_create": (_st, [_vp, _u32, _vp, UInt3, C.POINTER(_vp)]),
+    "fc_batch_destroy": (_st, [_vp]),
+    "fc_batch_num_particles": (_u32, [_vp]),
+    "fc_batch_scene_offsets": (_st, [_vp, _vp]),
+    "fc_batch_set_params": (_st, [_vp, _vp]),
+    "fc_batch_get_params": (_st, [_vp, _vp]),
+    "fc_batch_set_array": (_st, [_vp, C.c_int, _vp]),
+    "fc_batch_get_array": (_st, [_vp, C.c_int, _vp]),
+    "fc_batch_reset": (_st, [_vp, C.c_int, C.c_uint64, C.c_float, C.c_float]),
+    "fc_batch_update": (_st, [_vp, C.c_float]),
+    "fc_batch_run": (_st, [_vp, C.c_float, _u32]),
+    "fc_batch_stats_fp64": (_st, [_vp, _vp]),
+    "fc_batch_set_tuning": (_st, [_vp, C.POINTER(BatchTuning)]),
+    "fc_batch_get_tuning": (_st, [_vp, C.POINTER(BatchTuning)]),
+    "fc_batch_autotune": (_st, [_vp, C.c_float, _u32, C.POINTER(BatchTuning), _vp,
+                                C.POINTER(_u32)]),
 }
 
 

@@ -1,0 +1,550 @@
+// Batched independent scenes: many particle systems of uneven sizes stepped
+// together, one launch per stage for the whole batch.
+//
+// Scene s owns the particles [offsets[s], offsets[s+1]) of the concatenated
+// arrays and its own parameter block (spring, damping, radius, ...; every
+// scene uses the batch's grid dimensions).  The hash is scene-segmented:
+// key = s * C + cell with C = cells per scene (a power of two), so one
+// stable sort of (key, global id) orders the batch scene by scene and, inside
+// a scene, by (cell, id) exactly as the scene's own sort would.  The cell
+// tables are S * C entries, scene-major, cleared incrementally.  Each stage
+// therefore computes, per scene, exactly what ParticleSystem::update
+// (particleSystem.cpp:222-284) computes for that scene alone.
+#include "fc_internal.hpp"
+#include "fc_device.hpp"
+#include "radix_common.hpp"
+#include "collide.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+struct fc_batch {
+  fc_context* ctx = nullptr;
+  uint32_t S = 0;
+  uint32_t n = 0;
+  fc_uint3 grid{0, 0, 0};
+  uint32_t C = 0;
+  int cellBits = 0;
+  std::vector<uint32_t> offsets;  // S + 1
+  std::vector<fc_sim_params> params;
+  uint32_t* dOffsets = nullptr;
+  fc_sim_params* dParams = nullptr;
+  float* dContactD2 = nullptr;
+  float4* pos = nullptr;
+  float4* vel = nullptr;
+  float4* sortedPos = nullptr;
+  float4* sortedVel = nullptr;
+  uint32_t* hash = nullptr;
+  uint32_t* index = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* cellStart = nullptr;
+  uint32_t* cellEnd = nullptr;
+  double* stats = nullptr;  // S x 4
+  bool tablesTrackHash = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  float graphDt = 0.0f;
+  fc_batch_tuning tuning{4, 256, 0.0f};
+};
+
+namespace fc {
+
+namespace {
+
+constexpr int kBB = 256;  // block of the non-tuned batch kernels
+
+inline unsigned blocks_for(size_t n, int per) {
+  size_t g = (n + (size_t)per - 1) / (size_t)per;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+// Scene of particle i: the largest s with offsets[s] <= i (binary search).
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ offsets, uint32_t S,
+                                             uint32_t i) {
+  uint32_t lo = 0, hi = S;  // offsets[lo] <= i < offsets[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// integrate (box walls) + scene-segmented hash + digit-0 histogram, PER
+// particles per thread; clears the previous update's occupied cells.
+template <int PER>
+__global__ __launch_bounds__(kBB) void k_batch_integrate_hash(
+    const fc_sim_params* __restrict__ prm, const uint32_t* __restrict__ offsets, uint32_t S,
+    int cellBits, float4* __restrict__ pos, float4* __restrict__ vel, float dt,
+    uint32_t* __restrict__ keysOut, const uint32_t* __restrict__ prevSortedHash,
+    uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t n, int passes,
+    uint32_t* gHist, uint32_t* counters) {
+  __shared__ uint32_t s_hist[RS_RADIX];
+  for (int i = threadIdx.x; i < RS_RADIX; i += kBB) s_hist[i] = 0u;
+  histogram_producer_prologue(gHist, passes, counters);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const uint32_t i = blockIdx.x * (kBB * PER) + u * kBB + threadIdx.x;
+    uint32_t key = 0;
+    if (i < n) {
+      if (prevSortedHash != nullptr) {
+        const uint32_t h = prevSortedHash[i];
+        if (i == 0 || prevSortedHash[i - 1] != h) {
+          cellStart[h] = CELL_EMPTY;
+          cellEnd[h] = CELL_EMPTY;
+        }
+      }
+      const uint32_t s = scene_of(offsets, S, i);
+      const fc_sim_params P = prm[s];
+      const float4 p4 = pos[i];
+      const float4 v4 = vel[i];
+      v3f p = xyz(p4), v = xyz(v4);
+      integrate_box(P, P.particleRadius, dt, p, v);
+      pos[i] = make_float4(p.x, p.y, p.z, p4.w);
+      vel[i] = make_float4(v.x, v.y, v.z, v4.w);
+      key = (s << cellBits) | hash_of(P, p.x, p.y, p.z);
+      keysOut[i] = key;
+    }
+    histogram_add_digit(s_hist, key & (RS_RADIX - 1), i < n);
+  }
+  __syncthreads();
+  histogram_flush(s_hist, gHist, 0);
+}
+
+// collideD per scene: the scene comes from the sorted key, the cell tables
+// are the scene's slice of the batch tables.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_collide(
+    const fc_sim_params* __restrict__ prm, const float* __restrict__ contactD2, int cellBits,
+    uint32_t C, float4* __restrict__ newVel, const float4* __restrict__ sortedPos,
+    const float4* __restrict__ sortedVel, const uint32_t* __restrict__ hash,
+    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
+    const uint32_t* __restrict__ cellEnd, uint32_t n) {
+  __shared__ uint32_t s_list[kListCap][BS];
+  const uint32_t k =
+      xcd_grouped_block(blockIdx.x, gridDim.x, kCollideXcdGroup * (256 / BS)) * BS + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s = hash[k] >> cellBits;
+  const fc_sim_params P = prm[s];
+  const size_t base = (size_t)s * C;
+  const float4 p4 = sortedPos[k];
+  const v3f nv = collide_one_v3<false, BS>(P, contactD2[s], k, xyz(p4), xyz(sortedVel[k]),
+                                           P.particleRadius, sortedPos, sortedVel,
+                                           cellStart + base, cellEnd + base, n, s_list);
+  newVel[index[k]] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+}
+
+// Kinetic energy and momentum per scene in FP64, one block per scene, fixed
+// reduction order.
+__global__ __launch_bounds__(kBB) void k_batch_energy(const float4* __restrict__ vel,
+                                                      const uint32_t* __restrict__ offsets,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[4][kBB];
+  const uint32_t s = blockIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = offsets[s] + threadIdx.x; i < offsets[s + 1]; i += kBB) {
+    const float4 v = vel[i];
+    const double x = v.x, y = v.y, z = v.z;
+    acc[0] += 0.5 * (x * x + y * y + z * z);
+    acc[1] += x;
+    acc[2] += y;
+    acc[3] += z;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = kBB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[4 * s + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+hipError_t launch_batch_integrate_hash(fc_batch* b, float dt, const uint32_t* prev, int passes) {
+  fc_context* ctx = b->ctx;
+  const int per = b->tuning.integratePerThread;
+#define FC_BIH(PER)                                                                            \
+  hipLaunchKernelGGL(k_batch_integrate_hash<PER>, dim3(blocks_for(b->n, kBB * PER)), dim3(kBB), \
+                     0, ctx->stream, b->dParams, b->dOffsets, b->S, b->cellBits, b->pos, b->vel, \
+                     dt, b->keys, prev, b->cellStart, b->cellEnd, b->n, passes,                 \
+                     ctx->sort.histogram, ctx->sort.counters)
+  if (per == 1) FC_BIH(1);
+  else if (per == 2) FC_BIH(2);
+  else if (per == 8) FC_BIH(8);
+  else FC_BIH(4);
+#undef FC_BIH
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_collide(fc_batch* b) {
+  hipStream_t st = b->ctx->stream;
+  const int bs = b->tuning.collideBlock;
+#define FC_BC(BS)                                                                              \
+  hipLaunchKernelGGL(k_batch_collide<BS>, dim3(blocks_for(b->n, BS)), dim3(BS), 0, st,        \
+                     b->dParams, b->dContactD2, b->cellBits, b->C, b->vel, b->sortedPos,      \
+                     b->sortedVel, b->hash, b->index, b->cellStart, b->cellEnd, b->n)
+  if (bs == 64) FC_BC(64);
+  else if (bs == 128) FC_BC(128);
+  else FC_BC(256);
+#undef FC_BC
+  return hipGetLastError();
+}
+
+// S * C < 2^32 is checked at creation.
+int batch_key_bits(const fc_batch* b) { return key_bits_for_cells(b->S * b->C); }
+
+void drop_batch_graph(fc_batch* b) {
+  if (b->graphExec) hipGraphExecDestroy(b->graphExec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->graphExec = nullptr;
+  b->graph = nullptr;
+}
+
+// One update of every scene: integrate+hash -> sort -> reorder -> collide.
+fc_status enqueue_batch_update(fc_batch* b, float dt) {
+  fc_context* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const uint64_t cells = (uint64_t)b->S * b->C;
+  const uint32_t* prev = nullptr;
+  if (b->tablesTrackHash) {
+    prev = b->hash;
+  } else {
+    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells));
+    FC_HIP_TRY(launch_fill_u32(st, b->cellEnd, CELL_EMPTY, cells));
+  }
+  if (b->n == 0) return FC_SUCCESS;
+  const int bits = batch_key_bits(b);
+  FC_HIP_TRY(launch_batch_integrate_hash(b, dt, prev, radix_passes(bits, false)));
+  FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash, b->index,
+                              b->n, bits, nullptr, /*histogramReady=*/true));
+  FC_HIP_TRY(launch_reorder(st, b->cellStart, b->cellEnd, b->sortedPos, b->sortedVel, b->hash,
+                            b->index, b->pos, b->vel, b->n, (uint32_t)cells));
+  FC_HIP_TRY(launch_batch_collide(b));
+  b->tablesTrackHash = true;
+  return FC_SUCCESS;
+}
+
+fc_status capture_batch_graph(fc_batch* b, float dt) {
+  drop_batch_graph(b);
+  hipStream_t st = b->ctx->stream;
+  FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  fc_status s = enqueue_batch_update(b, dt);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != FC_SUCCESS) {
+    if (g) hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  b->graph = g;
+  e = hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_batch_graph(b);
+    return hip_error(e, "hipGraphInstantiate");
+  }
+  b->graphDt = dt;
+  return FC_SUCCESS;
+}
+
+fc_status upload_batch_params(fc_batch* b) {
+  std::vector<float> d2(b->S);
+  for (uint32_t s = 0; s < b->S; ++s)
+    d2[s] = contact_d2_threshold(b->params[s].particleRadius + b->params[s].particleRadius);
+  FC_HIP_TRY(hipMemcpyAsync(b->dParams, b->params.data(), b->S * sizeof(fc_sim_params),
+                            hipMemcpyHostToDevice, b->ctx->stream));
+  FC_HIP_TRY(hipMemcpyAsync(b->dContactD2, d2.data(), b->S * sizeof(float),
+                            hipMemcpyHostToDevice, b->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return FC_SUCCESS;
+}
+
+void free_batch(fc_batch* b) {
+  drop_batch_graph(b);
+  hipFree(b->dOffsets);
+  hipFree(b->dParams);
+  hipFree(b->dContactD2);
+  hipFree(b->pos);
+  hipFree(b->vel);
+  hipFree(b->sortedPos);
+  hipFree(b->sortedVel);
+  hipFree(b->hash);
+  hipFree(b->index);
+  hipFree(b->keys);
+  hipFree(b->cellStart);
+  hipFree(b->cellEnd);
+  hipFree(b->stats);
+}
+
+bool valid_tuning(const fc_batch_tuning& t) {
+  const int p = t.integratePerThread, c = t.collideBlock;
+  return (p == 1 || p == 2 || p == 4 || p == 8) && (c == 64 || c == 128 || c == 256);
+}
+
+}  // namespace
+
+}  // namespace fc
+
+using namespace fc;
+
+extern "C" {
+
+fc_status fc_batch_create(fc_context* ctx, uint32_t numScenes, const uint32_t* sceneCounts,
+                          fc_uint3 gridSize, fc_batch** out) {
+  if (!ctx || !sceneCounts || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *out = nullptr;
+  if (numScenes == 0) return set_error(FC_ERROR_INVALID_VALUE, "a batch needs a scene");
+  fc_sim_params P0 = default_params(0, gridSize);
+  FC_TRY(validate_params(P0));
+  if (gridSize.x < 4)
+    return set_error(FC_ERROR_INVALID_VALUE, "batched scenes need gridSize.x >= 4");
+  const uint64_t C = (uint64_t)gridSize.x * gridSize.y * gridSize.z;
+  if ((uint64_t)numScenes * C > 0xffffffffull)
+    return set_error(FC_ERROR_CAPACITY, "numScenes * cells per scene must be < 2^32");
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < numScenes; ++s) total += sceneCounts[s];
+  if (total >= (1ull << 27))
+    return set_error(FC_ERROR_CAPACITY, "a batch holds fewer than 2^27 particles");
+  fc_batch* b = new fc_batch();
+  b->ctx = ctx;
+  b->S = numScenes;
+  b->n = (uint32_t)total;
+  b->grid = gridSize;
+  b->C = (uint32_t)C;
+  b->cellBits = key_bits_for_cells(b->C);
+  b->offsets.resize(numScenes + 1);
+  b->offsets[0] = 0;
+  for (uint32_t s = 0; s < numScenes; ++s) b->offsets[s + 1] = b->offsets[s] + sceneCounts[s];
+  b->params.resize(numScenes);
+  for (uint32_t s = 0; s < numScenes; ++s) b->params[s] = default_params(sceneCounts[s], gridSize);
+  const size_t n = std::max<size_t>(b->n, 1);
+  const size_t cells = (size_t)numScenes * C;
+  hipError_t e = hipMalloc(&b->dOffsets, (numScenes + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->dParams, numScenes * sizeof(fc_sim_params));
+  if (e == hipSuccess) e = hipMalloc(&b->dContactD2, numScenes * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&b->pos, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&b->vel, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&b->sortedPos, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&b->sortedVel, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&b->hash, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->index, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->cellStart, cells * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->cellEnd, cells * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->stats, numScenes * 4 * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->dOffsets, b->offsets.data(), (numScenes + 1) * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b->pos, 0, n * sizeof(float4), ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(b->vel, 0, n * sizeof(float4), ctx->stream);
+  if (e != hipSuccess) {
+    free_batch(b);
+    delete b;
+    return hip_error(e, "batch allocation");
+  }
+  fc_status s = ensure_sort_capacity(ctx, b->n);
+  if (s == FC_SUCCESS) s = upload_batch_params(b);
+  if (s != FC_SUCCESS) {
+    free_batch(b);
+    delete b;
+    return s;
+  }
+  *out = b;
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_destroy(fc_batch* b) {
+  if (!b) return FC_SUCCESS;
+  if (b->ctx && b->ctx->stream) hipStreamSynchronize(b->ctx->stream);
+  free_batch(b);
+  delete b;
+  return FC_SUCCESS;
+}
+
+uint32_t fc_batch_num_particles(const fc_batch* b) { return b ? b->n : 0u; }
+
+fc_status fc_batch_scene_offsets(const fc_batch* b, uint32_t* offsets) {
+  if (!b || !offsets) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  std::memcpy(offsets, b->offsets.data(), (b->S + 1) * sizeof(uint32_t));
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_set_params(fc_batch* b, const fc_sim_params* params) {
+  if (!b || !params) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  for (uint32_t s = 0; s < b->S; ++s) {
+    FC_TRY(validate_params(params[s]));
+    if (params[s].gridSize.x != b->grid.x || params[s].gridSize.y != b->grid.y ||
+        params[s].gridSize.z != b->grid.z)
+      return set_error(FC_ERROR_INVALID_VALUE, "every scene must use the batch's grid size");
+  }
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  std::memcpy(b->params.data(), params, b->S * sizeof(fc_sim_params));
+  drop_batch_graph(b);
+  return upload_batch_params(b);
+}
+
+fc_status fc_batch_get_params(const fc_batch* b, fc_sim_params* params) {
+  if (!b || !params) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  std::memcpy(params, b->params.data(), b->S * sizeof(fc_sim_params));
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_set_array(fc_batch* b, fc_particle_array array, const float* data) {
+  if (!b || !data) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (array != FC_POSITION && array != FC_VELOCITY)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle array");
+  if (b->n == 0) return FC_SUCCESS;
+  FC_HIP_TRY(hipMemcpyAsync(array == FC_POSITION ? b->pos : b->vel, data,
+                            (size_t)b->n * sizeof(float4), hipMemcpyHostToDevice,
+                            b->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_get_array(fc_batch* b, fc_particle_array array, float* out) {
+  if (!b || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (array != FC_POSITION && array != FC_VELOCITY)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle array");
+  if (b->n == 0) return FC_SUCCESS;
+  FC_HIP_TRY(hipMemcpyAsync(out, array == FC_POSITION ? b->pos : b->vel,
+                            (size_t)b->n * sizeof(float4), hipMemcpyDeviceToHost,
+                            b->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return check_device_flags(b->ctx);
+}
+
+fc_status fc_batch_reset(fc_batch* b, fc_particle_config config, uint64_t seed, float jitterFrac,
+                         float velAmp) {
+  if (!b) return set_error(FC_ERROR_INVALID_VALUE, "batch is NULL");
+  if (config != FC_CONFIG_GRID && config != FC_CONFIG_RANDOM)
+    return set_error(FC_ERROR_INVALID_VALUE, "unknown particle config");
+  std::vector<float> pos((size_t)b->n * 4), vel((size_t)b->n * 4);
+  for (uint32_t s = 0; s < b->S; ++s) {
+    const uint32_t o = b->offsets[s], cnt = b->offsets[s + 1] - o;
+    generate_initial_state(pos.data() + (size_t)o * 4, vel.data() + (size_t)o * 4, cnt, config,
+                           seed + s, b->params[s].particleRadius, jitterFrac, velAmp);
+  }
+  FC_TRY(fc_batch_set_array(b, FC_POSITION, pos.data()));
+  return fc_batch_set_array(b, FC_VELOCITY, vel.data());
+}
+
+fc_status fc_batch_update(fc_batch* b, float deltaTime) {
+  if (!b) return set_error(FC_ERROR_INVALID_VALUE, "batch is NULL");
+  return enqueue_batch_update(b, deltaTime);
+}
+
+fc_status fc_batch_run(fc_batch* b, float deltaTime, uint32_t steps) {
+  if (!b) return set_error(FC_ERROR_INVALID_VALUE, "batch is NULL");
+  uint32_t i = 0;
+  if (steps > 0 && !b->tablesTrackHash) {
+    FC_TRY(enqueue_batch_update(b, deltaTime));
+    ++i;
+  }
+  if (i < steps && (!b->graphExec || b->graphDt != deltaTime))
+    FC_TRY(capture_batch_graph(b, deltaTime));
+  for (; i < steps; ++i) FC_HIP_TRY(hipGraphLaunch(b->graphExec, b->ctx->stream));
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_stats_fp64(fc_batch* b, double* out) {
+  if (!b || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  hipLaunchKernelGGL(k_batch_energy, dim3(b->S), dim3(kBB), 0, b->ctx->stream, b->vel,
+                     b->dOffsets, b->stats);
+  FC_HIP_TRY(hipGetLastError());
+  FC_HIP_TRY(hipMemcpyAsync(out, b->stats, (size_t)b->S * 4 * sizeof(double),
+                            hipMemcpyDeviceToHost, b->ctx->stream));
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_set_tuning(fc_batch* b, const fc_batch_tuning* t) {
+  if (!b || !t) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (!valid_tuning(*t))
+    return set_error(FC_ERROR_INVALID_VALUE,
+                     "integratePerThread must be 1, 2, 4 or 8 and collideBlock 64, 128 or 256");
+  FC_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  b->tuning.integratePerThread = t->integratePerThread;
+  b->tuning.collideBlock = t->collideBlock;
+  drop_batch_graph(b);
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_get_tuning(const fc_batch* b, fc_batch_tuning* t) {
+  if (!b || !t) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *t = b->tuning;
+  return FC_SUCCESS;
+}
+
+fc_status fc_batch_autotune(fc_batch* b, float deltaTime, uint32_t stepsPerCandidate,
+                            fc_batch_tuning* best, float* candidateMs, uint32_t* numCandidates) {
+  if (!b) return set_error(FC_ERROR_INVALID_VALUE, "batch is NULL");
+  if (stepsPerCandidate == 0) stepsPerCandidate = 3;
+  hipStream_t st = b->ctx->stream;
+  const size_t bytes = (size_t)std::max<uint32_t>(b->n, 1) * sizeof(float4);
+  // The trials step a copy of the state; the batch is restored afterwards.
+  float4 *savePos = nullptr, *saveVel = nullptr;
+  FC_HIP_TRY(hipMalloc(&savePos, bytes));
+  hipError_t e = hipMalloc(&saveVel, bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(savePos, b->pos, bytes, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(saveVel, b->vel, bytes, hipMemcpyDeviceToDevice, st);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  const int pers[4] = {1, 2, 4, 8};
+  const int blocks[3] = {64, 128, 256};
+  fc_batch_tuning keep = b->tuning, top = b->tuning;
+  float topMs = 0.0f;
+  uint32_t c = 0;
+  fc_status s = FC_SUCCESS;
+  for (int pi = 0; pi < 4 && e == hipSuccess && s == FC_SUCCESS; ++pi)
+    for (int bi = 0; bi < 3 && e == hipSuccess && s == FC_SUCCESS; ++bi, ++c) {
+      b->tuning.integratePerThread = pers[pi];
+      b->tuning.collideBlock = blocks[bi];
+      e = hipMemcpyAsync(b->pos, savePos, bytes, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(b->vel, saveVel, bytes, hipMemcpyDeviceToDevice, st);
+      b->tablesTrackHash = false;
+      if (e != hipSuccess) break;
+      s = enqueue_batch_update(b, deltaTime);  // warm-up (clears the tables)
+      if (s != FC_SUCCESS) break;
+      e = hipEventRecord(t0, st);
+      for (uint32_t k = 0; k < stepsPerCandidate && s == FC_SUCCESS; ++k)
+        s = enqueue_batch_update(b, deltaTime);
+      if (e == hipSuccess) e = hipEventRecord(t1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(t1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+      ms /= (float)stepsPerCandidate;
+      if (candidateMs) candidateMs[c] = ms;
+      if (c == 0 || ms < topMs) {
+        topMs = ms;
+        top = b->tuning;
+        top.stepMs = ms;
+      }
+    }
+  if (e == hipSuccess) e = hipMemcpyAsync(b->pos, savePos, bytes, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->vel, saveVel, bytes, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (t0) hipEventDestroy(t0);
+  if (t1) hipEventDestroy(t1);
+  hipFree(savePos);
+  hipFree(saveVel);
+  b->tablesTrackHash = false;
+  drop_batch_graph(b);
+  if (s != FC_SUCCESS) {
+    b->tuning = keep;
+    return s;
+  }
+  if (e != hipSuccess) {
+    b->tuning = keep;
+    return hip_error(e, "batch autotune");
+  }
+  b->tuning = top;
+  if (best) *best = top;
+  if (numCandidates) *numCandidates = c;
+  return FC_SUCCESS;
+}
+
+}  // extern "C"

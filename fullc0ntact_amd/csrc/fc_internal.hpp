@@ -171,6 +171,9 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 SortScratch& sort, int passes, uint32_t* devFlags);
 
 fc_status validate_params(const fc_sim_params& P);
+// Initial configurations of reset() into host float4 arrays (fc_system.hip).
+void generate_initial_state(float* pos, float* vel, uint32_t n, fc_particle_config config,
+                            uint64_t seed, float r, float jitterFrac, float velAmp);
 // Grows the context's sort scratch to hold n keys (synchronises first).
 fc_status ensure_sort_capacity(fc_context* ctx, size_t n);
 // Synchronises the context stream and converts the device error word.

@@ -209,6 +209,37 @@ fc_status capture_graph(fc_system* sys, float dt) {
 
 }  // namespace
 
+// reset(CONFIG_GRID / CONFIG_RANDOM) (particleSystem.cpp:380-444) into host
+// float4 arrays with a SplitMix64 stream: CONFIG_GRID is the s^3 lattice of
+// spacing 2r from -1+r (s = smallest integer with s^3 >= n) jittered by
+// U(-jitterFrac r, jitterFrac r) per axis; CONFIG_RANDOM is U(-1,1)^3;
+// velocities U(-velAmp, velAmp); pos.w = 1, vel.w = 0.
+void generate_initial_state(float* pos, float* vel, uint32_t n, fc_particle_config config,
+                            uint64_t seed, float r, float jitterFrac, float velAmp) {
+  SplitMix64 rng(seed);
+  if (config == FC_CONFIG_GRID) {
+    const uint32_t s = lattice_side(n);
+    const float spacing = r * 2.0f;
+    const float jitter = r * jitterFrac;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t x = i % s, y = (i / s) % s, z = i / (s * s);
+      const uint32_t c[3] = {x, y, z};
+      for (int a = 0; a < 3; ++a)
+        pos[(size_t)i * 4 + a] = ((spacing * (float)c[a] + r) - 1.0f) + rng.symmetric() * jitter;
+      pos[(size_t)i * 4 + 3] = 1.0f;
+      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
+      vel[(size_t)i * 4 + 3] = 0.0f;
+    }
+  } else {
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int a = 0; a < 3; ++a) pos[(size_t)i * 4 + a] = 2.0f * (rng.uniform01() - 0.5f);
+      pos[(size_t)i * 4 + 3] = 1.0f;
+      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
+      vel[(size_t)i * 4 + 3] = 0.0f;
+    }
+  }
+}
+
 }  // namespace fc
 
 using namespace fc;
@@ -349,29 +380,8 @@ fc_status fc_system_reset(fc_system* sys, fc_particle_config config, uint64_t se
   const uint32_t n = sys->n;
   if (n == 0) return FC_SUCCESS;
   std::vector<float> pos((size_t)n * 4), vel((size_t)n * 4);
-  SplitMix64 rng(seed);
-  const float r = sys->params.particleRadius;
-  if (config == FC_CONFIG_GRID) {
-    const uint32_t s = lattice_side(n);
-    const float spacing = r * 2.0f;
-    const float jitter = r * jitterFrac;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t x = i % s, y = (i / s) % s, z = i / (s * s);
-      const uint32_t c[3] = {x, y, z};
-      for (int a = 0; a < 3; ++a)
-        pos[(size_t)i * 4 + a] = ((spacing * (float)c[a] + r) - 1.0f) + rng.symmetric() * jitter;
-      pos[(size_t)i * 4 + 3] = 1.0f;
-      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
-      vel[(size_t)i * 4 + 3] = 0.0f;
-    }
-  } else {
-    for (uint32_t i = 0; i < n; ++i) {
-      for (int a = 0; a < 3; ++a) pos[(size_t)i * 4 + a] = 2.0f * (rng.uniform01() - 0.5f);
-      pos[(size_t)i * 4 + 3] = 1.0f;
-      for (int a = 0; a < 3; ++a) vel[(size_t)i * 4 + a] = rng.symmetric() * velAmp;
-      vel[(size_t)i * 4 + 3] = 0.0f;
-    }
-  }
+  generate_initial_state(pos.data(), vel.data(), n, config, seed, sys->params.particleRadius,
+                         jitterFrac, velAmp);
   FC_TRY(fc_system_set_array(sys, FC_POSITION, pos.data(), 0, (int)n));
   FC_TRY(fc_system_set_array(sys, FC_VELOCITY, vel.data(), 0, (int)n));
   return FC_SUCCESS;

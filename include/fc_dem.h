@@ -320,6 +320,59 @@ fc_status fc_contact_histogram(fc_context* ctx, const float* sortedPos, const ui
 fc_status fc_system_contact_histogram(fc_system* sys, uint32_t* hist, uint32_t nbins);
 
 /* ------------------------------------------------------------------------- */
+/* Batched independent scenes.  No counterpart in the reference: many        */
+/* ParticleSystem::update steps of independent scenes of uneven sizes run    */
+/* as one launch per stage over the whole batch.                              */
+/* ------------------------------------------------------------------------- */
+
+/* Scene s owns particles [offsets[s], offsets[s+1]) of the concatenated
+ * float4 arrays and its own fc_sim_params (every field may differ except
+ * gridSize/numCells, which are the batch's).  The hash is scene-segmented
+ * (key = s * cells + cell) so each scene's results are bit-identical to the
+ * same scene run alone as an fc_system.  Requires gridSize.x >= 4,
+ * numScenes * cells < 2^32 and fewer than 2^27 particles in total. */
+typedef struct fc_batch fc_batch;
+/* Launch shape of the tuned batch kernels. */
+typedef struct fc_batch_tuning {
+  int integratePerThread; /* particles per thread of integrate+hash: 1, 2, 4, 8 */
+  int collideBlock;       /* threads per block of the collide: 64, 128, 256     */
+  float stepMs;           /* measured step time of this choice (autotune)       */
+} fc_batch_tuning;
+
+fc_status fc_batch_create(fc_context* ctx, uint32_t numScenes, const uint32_t* sceneCounts,
+                          fc_uint3 gridSize, fc_batch** out);
+fc_status fc_batch_destroy(fc_batch* batch);
+uint32_t fc_batch_num_particles(const fc_batch* batch);
+/* numScenes + 1 offsets into the concatenated arrays. */
+fc_status fc_batch_scene_offsets(const fc_batch* batch, uint32_t* offsets);
+/* numScenes parameter blocks (defaults: the ParticleSystem constructor's). */
+fc_status fc_batch_set_params(fc_batch* batch, const fc_sim_params* params);
+fc_status fc_batch_get_params(const fc_batch* batch, fc_sim_params* params);
+/* All particles of all scenes, float4 each (host arrays). */
+fc_status fc_batch_set_array(fc_batch* batch, fc_particle_array array, const float* data);
+fc_status fc_batch_get_array(fc_batch* batch, fc_particle_array array, float* out);
+/* fc_system_reset per scene with seed + s and the scene's radius. */
+fc_status fc_batch_reset(fc_batch* batch, fc_particle_config config, uint64_t seed,
+                         float jitterFrac, float velAmp);
+/* One update of every scene. */
+fc_status fc_batch_update(fc_batch* batch, float deltaTime);
+/* steps updates, the steady step graph-launched; identical to repeated
+ * fc_batch_update. */
+fc_status fc_batch_run(fc_batch* batch, float deltaTime, uint32_t steps);
+/* Per scene {kinetic energy, px, py, pz} in binary64 (host double[4 *
+ * numScenes]). */
+fc_status fc_batch_stats_fp64(fc_batch* batch, double* out);
+/* Launch shapes; results do not depend on them. */
+fc_status fc_batch_set_tuning(fc_batch* batch, const fc_batch_tuning* tuning);
+fc_status fc_batch_get_tuning(const fc_batch* batch, fc_batch_tuning* tuning);
+/* Times every launch shape over stepsPerCandidate updates of a copy of the
+ * current state (restored afterwards), keeps the fastest and returns it;
+ * candidateMs (may be NULL) receives the 12 per-step times in the order
+ * integratePerThread {1,2,4,8} x collideBlock {64,128,256}. */
+fc_status fc_batch_autotune(fc_batch* batch, float deltaTime, uint32_t stepsPerCandidate,
+                            fc_batch_tuning* best, float* candidateMs, uint32_t* numCandidates);
+
+/* ------------------------------------------------------------------------- */
 /* Per-stage timing with HIP events on the context stream.                   */
 /* ------------------------------------------------------------------------- */
 
