@@ -7,6 +7,7 @@ CPU code.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from pathlib import Path
 
@@ -178,6 +179,20 @@ _SIGS = {
     "fc_batch_get_tuning": (_st, [_vp, C.POINTER(BatchTuning)]),
     "fc_batch_autotune": (_st, [_vp, C.c_float, _u32, C.POINTER(BatchTuning), _vp,
                                 C.POINTER(_u32)]),
+    "fc_sdf_num_params": (_u32, []),
+    "fc_sdf_param_stride": (_u32, []),
+    "fc_sdf_create": (_st, [_vp, _u32, _vp, _vp, C.c_float, C.c_uint64, C.POINTER(_vp)]),
+    "fc_sdf_destroy": (_st, [_vp]),
+    "fc_sdf_get_params": (_st, [_vp, _vp, _vp, _vp]),
+    "fc_sdf_set_params": (_st, [_vp, _vp]),
+    "fc_sdf_loss_grad": (_st, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "fc_sdf_train": (_st, [_vp, _vp, _vp, _u32, _u32, _u32, C.c_float, C.c_float, _vp]),
+    "fc_sdf_eval": (_st, [_vp, _vp, _vp, _u32, _vp, C.c_int]),
+    "fc_mfma_probe": (_st, [_vp, _vp, _vp, _vp]),
+    "fc_body_set_create": (_st, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, C.c_float, _vp,
+                                 C.POINTER(_vp)]),
+    "fc_body_set_destroy": (_st, [_vp]),
+    "fc_boundary_contacts": (_st, [_vp, _vp, _vp, _vp, _vp, _u32, C.c_float, _vp, _vp, _vp]),
 }
 
 
@@ -218,9 +233,20 @@ class Context:
         h = _vp()
         check(lib.fc_context_create(device, C.byref(h)))
         self.handle = h
+        self._dependents = weakref.WeakSet()
+
+    def register(self, obj) -> None:
+        """Objects created on this context (systems, maps, batches,
+        surrogates) are closed before the context is destroyed."""
+        self._dependents.add(obj)
 
     def close(self) -> None:
         if self.handle:
+            for obj in list(getattr(self, "_dependents", ())):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             check(lib.fc_context_destroy(self.handle))
             self.handle = None
 

@@ -21,6 +21,8 @@ class Batch:
         check(lib.fc_batch_create(self.ctx.handle, len(counts), counts.ctypes.data,
                                   _lib.UInt3(*[int(v) for v in gridSize]), C.byref(h)))
         self.handle = h
+        if context is not None:
+            context.register(self)
         self.num_scenes = len(counts)
         self.n = int(lib.fc_batch_num_particles(h))
         self.offsets = np.zeros(self.num_scenes + 1, np.uint32)

@@ -358,7 +358,8 @@ fc_status fc_batch_create(fc_context* ctx, uint32_t numScenes, const uint32_t* s
 
 fc_status fc_batch_destroy(fc_batch* b) {
   if (!b) return FC_SUCCESS;
-  if (b->ctx && b->ctx->stream) hipStreamSynchronize(b->ctx->stream);
+  // No use of b->ctx: the context may already be gone; hipFree waits for
+  // the device.
   free_batch(b);
   delete b;
   return FC_SUCCESS;

@@ -486,7 +486,8 @@ fc_status fc_distmap_build(fc_context* ctx, const double* vertices, uint32_t num
 
 fc_status fc_distmap_destroy(fc_distmap* m) {
   if (!m) return FC_SUCCESS;
-  if (m->ctx && m->ctx->stream) hipStreamSynchronize(m->ctx->stream);
+  // No use of m->ctx: the context may already be gone; hipFree waits for
+  // the device.
   hipFree(m->xs);
   hipFree(m->ys);
   hipFree(m->zs);

@@ -317,7 +317,8 @@ fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 grid
 
 fc_status fc_system_destroy(fc_system* sys) {
   if (!sys) return FC_SUCCESS;
-  if (sys->ctx && sys->ctx->stream) hipStreamSynchronize(sys->ctx->stream);
+  // No use of sys->ctx here: the context may already be gone.  hipFree
+  // waits for the device before releasing memory.
   drop_graph(sys);
   free_system_buffers(sys);
   delete sys;

@@ -27,6 +27,7 @@ class DistanceMap:
     def __init__(self, ctx: _lib.Context, handle: C.c_void_p):
         self.ctx = ctx
         self.handle = handle
+        ctx.register(self)
         cells = (C.c_int32 * 3)()
         cs = C.c_float()
         bmin = (C.c_float * 3)()

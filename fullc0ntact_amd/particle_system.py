@@ -39,6 +39,8 @@ class ParticleSystem:
         h = C.c_void_p()
         check(lib.fc_system_create(self.ctx.handle, int(numParticles), g, C.byref(h)))
         self.handle = h
+        if context is not None:
+            context.register(self)
         self.m_numParticles = int(numParticles)
         self.m_params = SimParams()
         check(lib.fc_system_get_params(self.handle, C.byref(self.m_params)))

@@ -109,6 +109,8 @@ typedef enum fc_boundary {
  * creates a context with its own HIP stream and the ParticleSystem default
  * parameters (particleSystem.cpp:48-68).  device < 0 keeps the current one. */
 fc_status fc_context_create(int device, fc_context** out);
+/* Objects created on a context (systems, maps, batches, surrogates) must not
+ * be used after it is destroyed; destroying them afterwards is safe. */
 fc_status fc_context_destroy(fc_context* ctx);
 /* Text of the last error raised on this thread (never NULL). */
 const char* fc_last_error(void);
@@ -373,6 +375,86 @@ fc_status fc_batch_autotune(fc_batch* batch, float deltaTime, uint32_t stepsPerC
                             fc_batch_tuning* best, float* candidateMs, uint32_t* numCandidates);
 
 /* ------------------------------------------------------------------------- */
+/* Learned signed-distance surrogate.  No counterpart in the reference: one  */
+/* small MLP per rigid body approximating its signed distance, trained on    */
+/* the exact distances of fc_mesh_signed_distance / a distance map, and     */
+/* evaluated with BF16 MFMA (FP32 accumulation) to cull far-field boundary   */
+/* queries before the exact distance-map query.                              */
+/* ------------------------------------------------------------------------- */
+
+/* Network per body (model-frame query q):
+ *   x = (q - center) / halfExtent
+ *   f = [sin(2 pi B x), cos(2 pi B x)], B = 16 x 3 fixed frequencies
+ *   h1 = SiLU(f W1 + b1), h2 = SiLU(h1 W2 + b2), h3 = SiLU(h2 W3 + b3)
+ *   distance = (h3 w4 + b4) * halfExtent
+ * with widths 32 -> 64 -> 64 -> 64 -> 1.  Parameters per body are
+ * fc_sdf_num_params() floats in the order W1[32][64] b1 W2[64][64] b2
+ * W3[64][64] b3 w4[64] b4 (W[input][output]), stored with a per-body stride
+ * of fc_sdf_param_stride() floats. */
+typedef struct fc_sdf fc_sdf;
+#define FC_SDF_FP32 0 /* FP32 forward (the training arithmetic) */
+#define FC_SDF_BF16 1 /* BF16 MFMA forward, FP32 accumulation   */
+
+uint32_t fc_sdf_num_params(void);
+uint32_t fc_sdf_param_stride(void);
+/* numBodies networks; centers (3 per body) and halfExtents normalise the
+ * inputs; frequencies ~ N(0, freqSigma^2) and Glorot-uniform weights from a
+ * SplitMix64 stream seeded with seed; zero biases. */
+fc_status fc_sdf_create(fc_context* ctx, uint32_t numBodies, const float* centers,
+                        const float* halfExtents, float freqSigma, uint64_t seed, fc_sdf** out);
+fc_status fc_sdf_destroy(fc_sdf* sdf);
+/* Host copies (any pointer may be NULL): params numBodies * stride floats,
+ * freqs numBodies * 48, frames numBodies * 4 (center xyz, halfExtent). */
+fc_status fc_sdf_get_params(fc_sdf* sdf, float* params, float* freqs, float* frames);
+fc_status fc_sdf_set_params(fc_sdf* sdf, const float* params);
+/* Gradient of L_b = 1/n sum_i (y_i - label_i / halfExtent_b)^2 for every
+ * body b over its samples (device arrays: points numBodies * samplesPerBody
+ * * 3 in the model frame, labels numBodies * samplesPerBody signed
+ * distances); grad (host, numBodies * stride) and loss (host, numBodies).
+ * FP32 forward/backward on the GPU, deterministic. */
+fc_status fc_sdf_loss_grad(fc_sdf* sdf, const float* points, const float* labels,
+                           uint32_t samplesPerBody, float* grad, float* loss);
+/* Adam (beta 0.9/0.999, eps 1e-8) over minibatches of batchPerBody
+ * consecutive samples of every body, all bodies in one launch per step; the
+ * learning rate is lr in the first epoch and multiplied by lrDecay after
+ * each; lossHistory (host, epochs * numBodies, may be NULL) receives each
+ * epoch's mean minibatch loss. */
+fc_status fc_sdf_train(fc_sdf* sdf, const float* points, const float* labels,
+                       uint32_t samplesPerBody, uint32_t batchPerBody, uint32_t epochs, float lr,
+                       float lrDecay, float* lossHistory);
+/* Signed distance of n model-frame points (device, 3 floats each) of the
+ * given bodies (device u32 per point; NULL = body 0) into out (device);
+ * precision FC_SDF_FP32 or FC_SDF_BF16.  A body id >= numBodies gives NaN. */
+fc_status fc_sdf_eval(fc_sdf* sdf, const uint32_t* bodies, const float* points, uint32_t n,
+                      float* out, int precision);
+
+/* Rigid bodies of a boundary.  Body b has a distance map in its model frame
+ * and a pose: model2world (row-major rotation) and the world position of
+ * its model origin.  A coarse lattice (lattice[3] cells of size latticeCell
+ * from latticeOrigin, cellBody[(z*ly + y)*lx + x] = body or -1) assigns at
+ * most one body to each region; a sphere is tested against the body of its
+ * lattice cell.  The maps must outlive the set. */
+typedef struct fc_body_set fc_body_set;
+fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distmap* const* maps,
+                             const float* model2world, const float* origins,
+                             const int32_t lattice[3], const float latticeOrigin[3],
+                             float latticeCell, const int32_t* cellBody, fc_body_set** out);
+fc_status fc_body_set_destroy(fc_body_set* set);
+/* Contacts of n spheres (device float4 positions, common radius) with the
+ * bodies: contact[i] = 1 when sphere i lies in its body's map box and the
+ * map distance (queryMap of the model-frame centre) is < radius, the test of
+ * checkDistMap (boundaries.cuh:38-89) per body.  With surrogate == NULL
+ * every sphere in a box gets the exact query.  With a surrogate (one
+ * network per body, same body order) the BF16 MFMA network first writes
+ * candidate[i] = 1 for spheres whose predicted distance is <= radius +
+ * margins[body] (host array, NULL = 0), and only those get the exact query.
+ * counts (host, may be NULL; synchronises): spheres in a box, exact queries,
+ * contacts. */
+fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrogate,
+                               const float* margins, const float* pos, uint32_t n, float radius,
+                               uint8_t* candidate, uint8_t* contact, uint32_t counts[3]);
+
+/* ------------------------------------------------------------------------- */
 /* Per-stage timing with HIP events on the context stream.                   */
 /* ------------------------------------------------------------------------- */
 
@@ -401,6 +483,10 @@ fc_status fc_context_get_timing(fc_context* ctx, fc_stage_timing* out, int max, 
  * profiler and measures the achievable copy bandwidth of the device. */
 fc_status fc_copy_probe(fc_context* ctx, void* dst, const void* src, size_t bytes, int iterations,
                         float* avgMs);
+/* Lane-layout check of the 16x16x32 BF16 MFMA as the surrogate kernels use
+ * it: d = a b for a 16x32 and b 32x16 row-major bf16 matrices, d 16x16
+ * row-major f32 (device arrays).  No counterpart in the reference. */
+fc_status fc_mfma_probe(fc_context* ctx, const uint16_t* a, const uint16_t* b, float* d);
 
 #ifdef __cplusplus
 }

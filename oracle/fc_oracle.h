@@ -157,6 +157,16 @@ void orc_integrate_distmap_ex(float* pos4, float* vel4, uint32_t n, float dt, co
  * boundary (m may be NULL: box walls). */
 void orc_update_ex(orc_state* s, float dt, const orc_params* P, const orc_distmap* m, int poly);
 
+/* ---- signed-distance surrogate MLP (fc_oracle_sdf.c) ---- */
+int orc_sdf_num_params(void);
+void orc_sdf_forward(const float* params, const float* freq, const float* frame,
+                     const float* pts, uint32_t n, double* out);
+void orc_sdf_loss_grad(const float* params, const float* freq, const float* frame,
+                       const float* pts, const float* labels, uint32_t n, double* grad,
+                       double* loss);
+void orc_sdf_forward_bf16(const float* params, const float* freq, const float* frame,
+                          const float* pts, uint32_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,10 @@ def _load() -> C.CDLL:
         "orc_contact_counts": (None, [vp, vp, vp, vp, u32, P, C.c_int]),
         "orc_integrate_distmap_ex": (None, [vp, vp, u32, f, P, C.POINTER(OrcDistMap), C.c_int]),
         "orc_update_ex": (None, [C.POINTER(OrcState), f, P, C.POINTER(OrcDistMap), C.c_int]),
+        "orc_sdf_num_params": (C.c_int, []),
+        "orc_sdf_forward": (None, [vp, vp, vp, vp, u32, vp]),
+        "orc_sdf_loss_grad": (None, [vp, vp, vp, vp, vp, u32, vp, vp]),
+        "orc_sdf_forward_bf16": (None, [vp, vp, vp, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -405,3 +409,35 @@ def reference_point_triangle():
         d = fn(*[_p(x) for x in arrs], _p(cp))
         return d, cp
     return call
+
+
+# ------------------------------------------------- signed-distance surrogate
+SDF_NPARAMS = 10497
+
+
+def _sdf_args(params, freq, frame, pts):
+    params = np.ascontiguousarray(params, np.float32).reshape(-1)[:SDF_NPARAMS]
+    freq = np.ascontiguousarray(freq, np.float32).reshape(48)
+    frame = np.ascontiguousarray(frame, np.float32).reshape(4)
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    return params, freq, frame, pts
+
+
+def sdf_forward(params, freq, frame, pts, bf16=False):
+    """Signed distance of model-frame points by the MLP of one body
+    (binary64; bf16=True rounds weights and activations as the MFMA kernel)."""
+    params, freq, frame, pts = _sdf_args(params, freq, frame, pts)
+    out = np.zeros(len(pts))
+    fn = lib.orc_sdf_forward_bf16 if bf16 else lib.orc_sdf_forward
+    fn(_p(params), _p(freq), _p(frame), _p(pts), len(pts), _p(out))
+    return out
+
+
+def sdf_loss_grad(params, freq, frame, pts, labels):
+    params, freq, frame, pts = _sdf_args(params, freq, frame, pts)
+    labels = np.ascontiguousarray(labels, np.float32).reshape(-1)
+    grad = np.zeros(SDF_NPARAMS)
+    loss = np.zeros(1)
+    lib.orc_sdf_loss_grad(_p(params), _p(freq), _p(frame), _p(pts), _p(labels), len(pts),
+                          _p(grad), _p(loss))
+    return grad, float(loss[0])
