@@ -150,6 +150,86 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int, progress_every: int = 0)
                 c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs)
 
 
+SDF_BODIES_LATTICE = 4        # 64 rigid bodies on a 4^3 lattice of [-1,1]^3
+SDF_SHAPES = 8                # seeded torus knots and superellipsoids
+SDF_SPHERES = 1 << 21         # 2,097,152 spheres
+SDF_RADIUS = 1.0 / 256
+SDF_MAP_CELLS = 256
+
+
+def sdf_leg(reps: int) -> dict:
+    """Boundary contacts of 2M spheres with 64 rigid bodies: the exact
+    distance-map query for every sphere in a body box, against the
+    surrogate path (BF16 MFMA network cull, exact query only in the band).
+    Spheres are uniform in [-1,1]^3 in random order, as the integrate stage
+    meets them once a bed has mixed."""
+    from fullc0ntact_amd import _lib
+    from fullc0ntact_amd.boundary import build_scene
+
+    ctx = _lib.Context()
+    log(f"[bench] sdf: building {SDF_SHAPES} shapes with {SDF_MAP_CELLS}^3 maps and networks")
+    sc = build_scene(ctx, lattice=SDF_BODIES_LATTICE, cells=SDF_MAP_CELLS, samples=1 << 16,
+                     epochs=30, seed=SEED, radius=SDF_RADIUS, shapes=SDF_SHAPES,
+                     knot_segments=(128, 8), ellipsoid_segments=(32, 16))
+    log(f"[bench] sdf: maps {sc.build_s:.1f} s, training + calibration {sc.train_s:.1f} s")
+    # spheres in the free space around the bodies: uniform draws whose
+    # centre lies inside a body (map distance < 0) are dropped
+    rng = np.random.default_rng(SEED)
+    draw = np.ones((SDF_SPHERES + SDF_SPHERES // 4, 4), np.float32)
+    draw[:, :3] = rng.uniform(-1.0, 1.0, (len(draw), 3))
+    dd = _lib.DeviceArray.from_host(ctx, draw)
+    inside, _ = sc.bodies.contacts(dd, len(draw), 0.0)
+    keep = inside.download(np.uint8, len(draw)) == 0
+    pos = np.ascontiguousarray(draw[keep][:SDF_SPHERES])
+    assert len(pos) == SDF_SPHERES
+    dp = _lib.DeviceArray.from_host(ctx, pos)
+    contact = _lib.DeviceArray(ctx, SDF_SPHERES)
+    cand = _lib.DeviceArray(ctx, SDF_SPHERES)
+    fe_dev, ce = sc.bodies.contacts(dp, SDF_SPHERES, SDF_RADIUS)
+    fe = fe_dev.download(np.uint8, SDF_SPHERES)
+    fs_dev, cs = sc.bodies.contacts(dp, SDF_SPHERES, SDF_RADIUS, sc.surrogate, sc.margins)
+    fs = fs_dev.download(np.uint8, SDF_SPHERES)
+    agreement = float(np.mean(fe == fs))
+    inter = int(np.sum(fe & fs))
+    union = int(np.sum(fe | fs))
+    times = {}
+    for name, sur in (("map", None), ("sdf", sc.surrogate)):
+        for _ in range(3):  # warm-up
+            sc.bodies.contacts(dp, SDF_SPHERES, SDF_RADIUS, sur, sc.margins if sur else None,
+                               cand, contact, counts=False)
+        ctx.sync()
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(reps):
+            sc.bodies.contacts(dp, SDF_SPHERES, SDF_RADIUS, sur, sc.margins if sur else None,
+                               cand, contact, counts=False)
+        t = ctx.timing()
+        ctx.enable_timing(False)
+        times[name] = {k: v[1] / max(v[0], 1) * 1e3 for k, v in t.items()}
+    t_map = times["map"]["boundary_exact"]
+    ts = times["sdf"]
+    t_sdf = ts["boundary_bucket"] + ts["boundary_cull"] + ts["boundary_exact_band"]
+    log(f"[bench] sdf: map path {t_map:.1f} us, surrogate path {t_sdf:.1f} us "
+        f"(bucket {ts['boundary_bucket']:.1f} + cull {ts['boundary_cull']:.1f} + band "
+        f"{ts['boundary_exact_band']:.1f}), agreement {agreement:.6f}")
+    out = {"bodies": SDF_BODIES_LATTICE ** 3, "shapes": SDF_SHAPES, "spheres": SDF_SPHERES,
+           "radius": SDF_RADIUS, "map_cells": SDF_MAP_CELLS,
+           "map_path_us": round(t_map, 2), "surrogate_path_us": round(t_sdf, 2),
+           "bucket_us": round(ts["boundary_bucket"], 2),
+           "cull_us": round(ts["boundary_cull"], 2),
+           "band_exact_us": round(ts["boundary_exact_band"], 2),
+           "speedup": round(t_map / t_sdf, 3), "contact_agreement": agreement,
+           "contact_set_jaccard": (inter / union) if union else 1.0,
+           "in_box": int(ce[0]), "exact_queries_surrogate": int(cs[1]), "contacts": int(ce[2]),
+           "map_build_s": round(sc.build_s, 1), "train_s": round(sc.train_s, 1)}
+    sc.bodies.close()
+    sc.surrogate.close()
+    for m in sc.shape_maps:
+        m.close()
+    ctx.close()
+    return out
+
+
 def cpu_leg(target_seconds: float, threads: int) -> dict:
     from oracle import oracle as orc
 
@@ -176,6 +256,8 @@ def main(argv=None) -> int:
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-sdf", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--sdf-reps", type=int, default=20)
     ap.add_argument("--progress-every", type=int, default=0,
                     help="run the warmup in chunks of this many steps, logging each")
     a = ap.parse_args(argv)
@@ -211,6 +293,10 @@ def main(argv=None) -> int:
                 "kernel": dom, "bytes_per_launch": nbytes, "avg_launch_us": us,
                 "copy_probe_GBs": round(g["copy_gbs"], 1)}
 
+    sdf = None
+    if not a.no_sdf:
+        sdf = sdf_leg(a.sdf_reps)
+
     cpu = None
     if not a.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -241,6 +327,7 @@ def main(argv=None) -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_us": {"run": run_us, "update": upd_us},
+        "sdf_vs_distance_map": sdf,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -17,11 +17,12 @@ from .sdf import BF16, SdfSurrogate, training_samples
 
 class BodySet:
     def __init__(self, ctx: _lib.Context, maps, model2world, origins, lattice, lattice_origin,
-                 lattice_cell: float, cell_body):
+                 lattice_cell: float, cell_body, body_net=None):
         self.ctx = ctx
         self.maps = list(maps)
         B = len(self.maps)
         handles = (C.c_void_p * B)(*[m.handle.value for m in self.maps])
+        net = np.ascontiguousarray(body_net if body_net is not None else np.arange(B), np.int32)
         m2w = np.ascontiguousarray(model2world, np.float32).reshape(B, 9)
         org = np.ascontiguousarray(origins, np.float32).reshape(B, 3)
         lat = np.ascontiguousarray(lattice, np.int32).reshape(3)
@@ -29,8 +30,9 @@ class BodySet:
         cb = np.ascontiguousarray(cell_body, np.int32).reshape(-1)
         h = C.c_void_p()
         check(lib.fc_body_set_create(ctx.handle, B, handles, m2w.ctypes.data, org.ctypes.data,
-                                     lat.ctypes.data, lo.ctypes.data, C.c_float(lattice_cell),
-                                     cb.ctypes.data, C.byref(h)))
+                                     net.ctypes.data, lat.ctypes.data, lo.ctypes.data,
+                                     C.c_float(lattice_cell), cb.ctypes.data, C.byref(h)))
+        self.body_net = net
         self.handle = h
         ctx.register(self)
         self.num_bodies = B
@@ -82,10 +84,13 @@ def rotation(rng) -> np.ndarray:
 class Scene:
     bodies: BodySet
     surrogate: SdfSurrogate
-    margins: np.ndarray
-    meshes: list
-    maps: list
-    frames: np.ndarray  # per body: center xyz, halfExtent (model frame)
+    margins: np.ndarray  # per body
+    meshes: list         # per shape
+    maps: list           # per body (shared by the bodies of a shape)
+    frames: np.ndarray   # per shape: center xyz, halfExtent (model frame)
+    shape_maps: list = None
+    build_s: float = 0.0
+    train_s: float = 0.0
 
 
 def body_mesh(kind: str, rng, size: float):
@@ -103,48 +108,68 @@ def body_mesh(kind: str, rng, size: float):
 
 def build_scene(ctx: _lib.Context, lattice: int = 4, cells: int = 64, samples: int = 1 << 16,
                 epochs: int = 30, seed: int = 1973, domain=(-1.0, 1.0),
-                radius: float = 1.0 / 256) -> Scene:
-    """lattice^3 rigid bodies, one per lattice cell of [domain]^3, seeded
-    torus knots and superellipsoids with random rotations; per body a
-    cells^3 distance map over its box and a surrogate network trained on
-    `samples` exact mesh distances; margins calibrated so that the culled
+                radius: float = 1.0 / 256, shapes: int | None = None,
+                knot_segments=(256, 12), ellipsoid_segments=(48, 24)) -> Scene:
+    """lattice^3 rigid bodies, one per lattice cell of [domain]^3, with
+    random rotations, instancing `shapes` seeded shapes (torus knots and
+    superellipsoids; default one shape per body): per shape a cells^3
+    distance map over its box and a surrogate network trained on `samples`
+    exact mesh distances; per-body margins calibrated so that the culled
     query keeps every contact of the exact one on held-out points."""
+    import time
     rng = np.random.default_rng(seed)
     lo, hi = domain
     cell = (hi - lo) / lattice
     B = lattice ** 3
-    mesh_list, maps, frames, m2w, origins = [], [], [], [], []
-    for b in range(B):
-        iz, iy, ix = b // (lattice * lattice), (b // lattice) % lattice, b % lattice
-        centre = np.array([lo + (ix + 0.5) * cell, lo + (iy + 0.5) * cell, lo + (iz + 0.5) * cell])
-        v, f = body_mesh("knot" if b % 2 == 0 else "ellipsoid", rng, 0.6 * cell)
+    S = B if shapes is None else int(shapes)
+    t0 = time.perf_counter()
+    mesh_list, shape_maps, frames = [], [], []
+    for s in range(S):
+        if s % 2 == 0:
+            p, q = [(2, 3), (3, 2), (2, 5), (3, 4)][rng.integers(0, 4)]
+            size = 0.6 * cell
+            v, f = meshes.torus_knot(p, q, knot_segments[0], knot_segments[1],
+                                     radius=0.05 * size / 0.3,
+                                     scale=size / 3.2 * rng.uniform(0.8, 1.0))
+        else:
+            a3 = 0.6 * cell * rng.uniform(0.25, 0.45, 3)
+            v, f = meshes.superellipsoid(a3[0], a3[1], a3[2], rng.uniform(0.3, 1.2),
+                                         rng.uniform(0.3, 1.2), *ellipsoid_segments)
         mesh_list.append((v, f))
-        # model-frame box: cube around the mesh, padded
         mn, mx = v.min(0), v.max(0)
         c = 0.5 * (mn + mx)
         half = 0.5 * float(np.max(mx - mn)) * 1.15
         cs = 2.0 * half / cells
-        maps.append(DistanceMap.from_mesh(ctx, v, f, (cells, cells, cells), cs, c - half))
+        shape_maps.append(DistanceMap.from_mesh(ctx, v, f, (cells, cells, cells), cs, c - half))
         frames.append([*c, half])
-        R = rotation(rng)
-        m2w.append(R.reshape(-1))
-        origins.append(centre)
+    ctx.sync()
+    build_s = time.perf_counter() - t0
     frames = np.array(frames, np.float32)
+    m2w, origins, body_net = [], [], []
+    for b in range(B):
+        iz, iy, ix = b // (lattice * lattice), (b // lattice) % lattice, b % lattice
+        origins.append([lo + (ix + 0.5) * cell, lo + (iy + 0.5) * cell, lo + (iz + 0.5) * cell])
+        m2w.append(rotation(rng).reshape(-1))
+        body_net.append(b % S)
+    maps = [shape_maps[k] for k in body_net]
     bodies = BodySet(ctx, maps, np.array(m2w), np.array(origins), (lattice,) * 3, (lo,) * 3,
-                     cell, np.arange(B, dtype=np.int32))
+                     cell, np.arange(B, dtype=np.int32), np.array(body_net, np.int32))
+    t0 = time.perf_counter()
     sur = SdfSurrogate(ctx, frames[:, :3], frames[:, 3], freq_sigma=1.0, seed=seed)
-    pts = np.zeros((B, samples, 3), np.float32)
-    lab = np.zeros((B, samples), np.float32)
-    for b, (v, f) in enumerate(mesh_list):
-        p = training_samples(v, f, samples, frames[b, :3], float(frames[b, 3]), 0.6, 0.04,
-                             seed=seed + 17 * b)
+    pts = np.zeros((S, samples, 3), np.float32)
+    lab = np.zeros((S, samples), np.float32)
+    for s, (v, f) in enumerate(mesh_list):
+        p = training_samples(v, f, samples, frames[s, :3], float(frames[s, 3]), 0.6, 0.04,
+                             seed=seed + 17 * s)
         sd, _ = mesh_signed_distance(ctx, v, f, p)
-        pts[b] = p
-        lab[b] = sd
+        pts[s] = p
+        lab[s] = sd
     dp, dl = DeviceArray.from_host(ctx, pts), DeviceArray.from_host(ctx, lab)
     sur.train(dp, dl, samples, 1024, epochs, 3e-3, 0.97)
-    margins = calibrate_margins(ctx, sur, maps, mesh_list, frames, radius, rng)
-    return Scene(bodies, sur, margins, mesh_list, maps, frames)
+    shape_margins = calibrate_margins(ctx, sur, shape_maps, mesh_list, frames, radius, rng)
+    train_s = time.perf_counter() - t0
+    margins = shape_margins[np.array(body_net)]
+    return Scene(bodies, sur, margins, mesh_list, maps, frames, shape_maps, build_s, train_s)
 
 
 def calibrate_margins(ctx, sur: SdfSurrogate, maps, mesh_list, frames, radius: float, rng,

@@ -38,6 +38,7 @@ namespace fc {
 struct BodyDev {
   float w2m[9];
   float origin[3];
+  int net;  // surrogate network of the body's shape
   DistMapView map;
 };
 }  // namespace fc
@@ -49,9 +50,14 @@ struct fc_body_set {
   float latticeOrigin[3] = {0, 0, 0};
   float latticeCell = 0.0f;
   fc::BodyDev* bodies = nullptr;  // device, B
+  int32_t maxNet = 0;              // largest network index of a body
   int32_t* cellBody = nullptr;    // device, lattice cells
   float* margins = nullptr;       // device, B
   uint32_t* counters = nullptr;   // device, 3
+  uint16_t* netOf = nullptr;      // device, capacity: bucket of each sphere
+  uint32_t* order = nullptr;      // device, capacity: spheres in bucket order
+  uint32_t* bins = nullptr;       // device, 2 * (maxNet + 3): counts, cursors
+  size_t capacity = 0;
 };
 
 namespace fc {
@@ -59,6 +65,23 @@ namespace fc {
 namespace {
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
+
+// Eight floats rounded to bfloat16 (round to nearest even, packed
+// conversions) as one MFMA operand register group.
+__device__ __forceinline__ bf16x8 pack8(float a0, float a1, float a2, float a3, float a4, float a5,
+                                        float a6, float a7) {
+  bf16v8 v;
+  v[0] = (__bf16)a0;
+  v[1] = (__bf16)a1;
+  v[2] = (__bf16)a2;
+  v[3] = (__bf16)a3;
+  v[4] = (__bf16)a4;
+  v[5] = (__bf16)a5;
+  v[6] = (__bf16)a6;
+  v[7] = (__bf16)a7;
+  return __builtin_bit_cast(bf16x8, v);
+}
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 64;  // samples per training tile
@@ -66,6 +89,11 @@ constexpr int kTB = 256;   // training / inference block
 constexpr int kWPad = 10500;
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// SiLU of the inference path: the hardware exp2 and reciprocal (about 1 ulp
+// each) instead of the IEEE division of the training arithmetic.
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
 __device__ __forceinline__ float silu(float z) { return z * sigmoidf_(z); }
 __device__ __forceinline__ float silu_grad(float z) {
   const float s = sigmoidf_(z);
@@ -438,16 +466,9 @@ __device__ __forceinline__ bf16x8 load_frag(const uint16_t* __restrict__ packed,
 // and SiLU): elements 0..3 from the first, 4..7 from the second.
 __device__ __forceinline__ bf16x8 act_operand(const floatx4& lo, const floatx4& hi,
                                               const float4& blo, const float4& bhi) {
-  bf16x8 r;
-  r[0] = (short)bf16_rne(silu(lo[0] + blo.x));
-  r[1] = (short)bf16_rne(silu(lo[1] + blo.y));
-  r[2] = (short)bf16_rne(silu(lo[2] + blo.z));
-  r[3] = (short)bf16_rne(silu(lo[3] + blo.w));
-  r[4] = (short)bf16_rne(silu(hi[0] + bhi.x));
-  r[5] = (short)bf16_rne(silu(hi[1] + bhi.y));
-  r[6] = (short)bf16_rne(silu(hi[2] + bhi.z));
-  r[7] = (short)bf16_rne(silu(hi[3] + bhi.w));
-  return r;
+  return pack8(silu_fast(lo[0] + blo.x), silu_fast(lo[1] + blo.y), silu_fast(lo[2] + blo.z),
+               silu_fast(lo[3] + blo.w), silu_fast(hi[0] + bhi.x), silu_fast(hi[1] + bhi.y),
+               silu_fast(hi[2] + bhi.z), silu_fast(hi[3] + bhi.w));
 }
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -460,15 +481,13 @@ __device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
                                                const float* __restrict__ P,
                                                const float* __restrict__ F, float x, float y,
                                                float z, int g) {
-  bf16x8 bop;
+  float sv[4], cv[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int k = 4 * g + j;
-    float s, c;
-    sincos_rev(F[3 * k] * x + F[3 * k + 1] * y + F[3 * k + 2] * z, s, c);
-    bop[j] = (short)bf16_rne(s);
-    bop[4 + j] = (short)bf16_rne(c);
+    sincos_rev(F[3 * k] * x + F[3 * k + 1] * y + F[3 * k + 2] * z, sv[j], cv[j]);
   }
+  const bf16x8 bop = pack8(sv[0], sv[1], sv[2], sv[3], cv[0], cv[1], cv[2], cv[3]);
   floatx4 d[4];
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob)
@@ -491,10 +510,10 @@ __device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
   for (int ob = 0; ob < 4; ++ob) {
     const float4 bb = ld4(P + kSdfOffB3 + ob * 16 + 4 * g);
     const float4 ww = ld4(P + kSdfOffW4 + ob * 16 + 4 * g);
-    yv += silu(d[ob][0] + bb.x) * ww.x;
-    yv += silu(d[ob][1] + bb.y) * ww.y;
-    yv += silu(d[ob][2] + bb.z) * ww.z;
-    yv += silu(d[ob][3] + bb.w) * ww.w;
+    yv += silu_fast(d[ob][0] + bb.x) * ww.x;
+    yv += silu_fast(d[ob][1] + bb.y) * ww.y;
+    yv += silu_fast(d[ob][2] + bb.z) * ww.z;
+    yv += silu_fast(d[ob][3] + bb.w) * ww.w;
   }
   yv += __shfl_xor(yv, 16);
   yv += __shfl_xor(yv, 32);
@@ -517,9 +536,13 @@ __global__ __launch_bounds__(kTB) void k_sdf_infer(const uint16_t* __restrict__ 
   const uint32_t wave = (blockIdx.x * kTB + threadIdx.x) >> 6;
   const uint32_t nWaves = (gridDim.x * kTB) >> 6;
   const uint32_t tiles = (n + 15) / 16;
+  // contiguous runs of tiles per wave: consecutive tiles share a network,
+  // so the weight fragments are reloaded only where the network changes
+  const uint32_t per = (tiles + nWaves - 1) / nWaves;
+  const uint32_t t0 = wave * per, t1 = t0 + per < tiles ? t0 + per : tiles;
   bf16x8 frag[kSdfFrags];
   int cur = -1;
-  for (uint32_t tile = wave; tile < tiles; tile += nWaves) {
+  for (uint32_t tile = t0; tile < t1; ++tile) {
     const uint32_t q = tile * 16 + col;
     const bool valid = q < n;
     uint32_t body = valid ? (bodies ? bodies[q] : 0u) : 0u;
@@ -614,36 +637,140 @@ struct LatticeArg {
   float ox, oy, oz, cell;
 };
 
-// Surrogate cull: candidate[i] = 1 when sphere i lies in its body's map box
-// and the BF16 network predicts a distance <= r + margin[body].
+// Spheres are first grouped by the network of their body (one bucket per
+// network plus one for spheres outside every box) so that each 16-sphere
+// MFMA tile of the cull sees one network: count, scan, scatter.  The order
+// inside a bucket is arbitrary; every sphere's result depends only on its
+// own column of the tile.
+// Adds `valid` lanes' values v to an LDS histogram with one atomic per
+// distinct value per wave; returns each lane's rank among the wave's lanes
+// of the same value plus the count already in the bin (the lane's slot).
+__device__ __forceinline__ uint32_t wave_hist_add(uint32_t* s_cnt, uint32_t v, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t todo = __ballot(valid);
+  uint32_t slot = 0;
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t lv = (uint32_t)__shfl((int)v, leader);
+    const uint64_t same = __ballot(valid && v == lv);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&s_cnt[lv], (uint32_t)__popcll(same));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (valid && v == lv) slot = base + (uint32_t)__popcll(same & below);
+    todo &= ~same;
+  }
+  return slot;
+}
+
+// Spheres per block of the bucketing kernels (16 per thread): few blocks,
+// so few same-address global atomics per bucket.
+constexpr uint32_t kBucketChunk = 16 * kTB;
+
+__global__ __launch_bounds__(kTB) void k_bucket_count(
+    const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
+    uint32_t B, const float4* __restrict__ pos, uint32_t n, uint32_t nets,
+    uint16_t* __restrict__ netOf, uint32_t* __restrict__ counts) {
+  extern __shared__ uint32_t s_cnt[];
+  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) s_cnt[k] = 0u;
+  __syncthreads();
+  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {
+    const uint32_t i = blockIdx.x * kBucketChunk + u * kTB + threadIdx.x;
+    uint32_t net = nets;  // nets = outside every box
+    if (i < n) {
+      v3f q;
+      const int b = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B,
+                            xyz(pos[i]), q);
+      if (b >= 0 && in_box(bodies[b].map, q)) net = (uint32_t)bodies[b].net;
+      netOf[i] = (uint16_t)net;
+    }
+    wave_hist_add(s_cnt, net, i < n);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k <= nets; k += kTB)
+    if (s_cnt[k]) atomicAdd(&counts[k], s_cnt[k]);
+}
+
+// Exclusive scan of the nets + 1 bucket counts (one block): cursor[k] =
+// start of bucket k; cursor[nets + 1] = n.
+__global__ __launch_bounds__(kTB) void k_bucket_scan(const uint32_t* __restrict__ counts,
+                                                     uint32_t nets, uint32_t* __restrict__ cursor) {
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k <= nets; ++k) {
+      cursor[k] = acc;
+      acc += counts[k];
+    }
+    cursor[nets + 1] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kTB) void k_bucket_scatter(const uint16_t* __restrict__ netOf,
+                                                        uint32_t n, uint32_t nets,
+                                                        uint32_t* __restrict__ cursor,
+                                                        uint32_t* __restrict__ order) {
+  extern __shared__ uint32_t s_cnt[];
+  uint32_t* s_base = s_cnt + nets + 1;
+  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) s_cnt[k] = 0u;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kBucketChunk;
+  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {  // this block's counts
+    const uint32_t i = base + u * kTB + threadIdx.x;
+    wave_hist_add(s_cnt, i < n ? (uint32_t)netOf[i] : 0u, i < n);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) {
+    s_base[k] = s_cnt[k] ? atomicAdd(&cursor[k], s_cnt[k]) : 0u;
+    s_cnt[k] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {  // slots
+    const uint32_t i = base + u * kTB + threadIdx.x;
+    const uint32_t net = i < n ? (uint32_t)netOf[i] : 0u;
+    const uint32_t slot = wave_hist_add(s_cnt, net, i < n);
+    if (i < n) order[s_base[net] + slot] = i;
+  }
+}
+
+// Surrogate cull over the bucketed spheres: the BF16 network predicts each
+// in-box sphere's distance; candidate[i] = 1 for those within radius +
+// margin[body] (0 for every other sphere).
 __global__ __launch_bounds__(kTB) void k_boundary_cull(
     const uint16_t* __restrict__ packed, const float* __restrict__ params,
     const float* __restrict__ freq, const float* __restrict__ frame,
     const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
-    uint32_t B, const float* __restrict__ margins, const float4* __restrict__ pos, uint32_t n,
-    float radius, uint8_t* __restrict__ candidate) {
+    uint32_t B, const float* __restrict__ margins, const float4* __restrict__ pos,
+    const uint32_t* __restrict__ order, uint32_t n, float radius,
+    uint8_t* __restrict__ candidate) {
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const uint32_t wave = (blockIdx.x * kTB + threadIdx.x) >> 6;
   const uint32_t nWaves = (gridDim.x * kTB) >> 6;
   const uint32_t tiles = (n + 15) / 16;
+  // contiguous runs of tiles per wave: consecutive tiles share a network,
+  // so the weight fragments are reloaded only where the network changes
+  const uint32_t per = (tiles + nWaves - 1) / nWaves;
+  const uint32_t t0 = wave * per, t1 = t0 + per < tiles ? t0 + per : tiles;
   bf16x8 frag[kSdfFrags];
   int cur = -1;
-  for (uint32_t tile = wave; tile < tiles; tile += nWaves) {
-    const uint32_t i = tile * 16 + col;
-    int body = -1;
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    const uint32_t slot = tile * 16 + col;
+    uint32_t i = 0;
+    int body = -1, net = -1;
     v3f q = make_v3(0.0f, 0.0f, 0.0f);
-    if (i < n) {
-      const float4 p4 = pos[i];
-      body = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B, xyz(p4), q);
+    if (slot < n) {
+      i = order[slot];
+      body = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B,
+                     xyz(pos[i]), q);
       if (body >= 0 && !in_box(bodies[body].map, q)) body = -1;
+      if (body >= 0) net = bodies[body].net;
     }
-    uint8_t flag = 0;
+    bool cand = false;
     bool remaining = body >= 0;
     while (true) {
       const uint64_t mask = __ballot(remaining);
       if (mask == 0) break;
       const int leadLane = __ffsll((unsigned long long)mask) - 1;
-      const int lead = __shfl(body, leadLane);
+      const int lead = __shfl(net, leadLane);
       if (lead != cur) {
 #pragma unroll
         for (int f = 0; f < kSdfFrags; ++f) frag[f] = load_frag(packed, (uint32_t)lead, f, lane);
@@ -655,11 +782,11 @@ __global__ __launch_bounds__(kTB) void k_boundary_cull(
       float x, y, z;
       sdf_norm(fr, q.x, q.y, q.z, x, y, z);
       const float yv = sdf_tile_mfma(frag, P, F, x, y, z, g);
-      const bool mine = remaining && body == lead;
-      if (mine) flag = (yv * fr[3] <= radius + margins[lead]) ? 1 : 0;
+      const bool mine = remaining && net == lead;
+      if (mine) cand = yv * fr[3] <= radius + margins[body];
       remaining = remaining && !mine;
     }
-    if (i < n && g == 0) candidate[i] = flag;
+    if (slot < n && g == 0) candidate[i] = cand ? 1 : 0;
   }
 }
 
@@ -740,6 +867,16 @@ void free_sdf(fc_sdf* m) {
 }
 
 constexpr uint32_t kTilesPerBlock = 8;
+
+// Blocks (of 4 waves) of the MFMA kernels for n queries: up to 3 resident
+// waves per SIMD of the 256 CUs, each wave taking at least 16 tiles.
+uint32_t mfma_blocks(uint32_t n) {
+  const uint32_t tiles = (n + 15) / 16;
+  uint32_t waves = (tiles + 15) / 16;
+  if (waves > 3072u) waves = 3072u;
+  const uint32_t blocks = (waves + 3) / 4;
+  return blocks == 0 ? 1u : blocks;
+}
 
 // One gradient evaluation (+ optional Adam step) over samples
 // [batchStart, batchStart + batchSize) of every body.
@@ -938,9 +1075,7 @@ fc_status fc_sdf_eval(fc_sdf* m, const uint32_t* bodies, const float* points, ui
                        m->freq, m->frame, bodies, m->B, points, n, out);
   } else if (precision == FC_SDF_BF16) {
     FC_TRY(sdf_pack(m));
-    const uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = (tiles + 3) / 4;
-    if (blocks > 2048u) blocks = 2048u;
+    const uint32_t blocks = mfma_blocks(n);
     hipLaunchKernelGGL(k_sdf_infer, dim3(blocks), dim3(kTB), 0, st, m->packed, m->params,
                        m->freq, m->frame, bodies, m->B, points, n, out);
   } else {
@@ -952,8 +1087,9 @@ fc_status fc_sdf_eval(fc_sdf* m, const uint32_t* bodies, const float* points, ui
 
 fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distmap* const* maps,
                              const float* model2world, const float* origins,
-                             const int32_t lattice[3], const float latticeOrigin[3],
-                             float latticeCell, const int32_t* cellBody, fc_body_set** out) {
+                             const int32_t* bodyNet, const int32_t lattice[3],
+                             const float latticeOrigin[3], float latticeCell,
+                             const int32_t* cellBody, fc_body_set** out) {
   if (!ctx || !maps || !model2world || !origins || !lattice || !latticeOrigin || !cellBody || !out)
     return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
   *out = nullptr;
@@ -962,8 +1098,12 @@ fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distm
     return set_error(FC_ERROR_INVALID_VALUE, "invalid body lattice");
   const size_t cells = (size_t)lattice[0] * lattice[1] * lattice[2];
   std::vector<BodyDev> host(numBodies);
+  int32_t maxNet = 0;
   for (uint32_t b = 0; b < numBodies; ++b) {
     if (!maps[b]) return set_error(FC_ERROR_INVALID_VALUE, "a body has no distance map");
+    host[b].net = bodyNet ? bodyNet[b] : (int)b;
+    if (host[b].net < 0) return set_error(FC_ERROR_INVALID_VALUE, "negative network index");
+    maxNet = std::max(maxNet, host[b].net);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) host[b].w2m[3 * i + j] = model2world[9 * b + 3 * j + i];
     for (int a = 0; a < 3; ++a) host[b].origin[a] = origins[3 * b + a];
@@ -980,6 +1120,7 @@ fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distm
     s->latticeOrigin[a] = latticeOrigin[a];
   }
   s->latticeCell = latticeCell;
+  s->maxNet = maxNet;
   hipStream_t st = ctx->stream;
   hipError_t e = hipMalloc(&s->bodies, numBodies * sizeof(BodyDev));
   if (e == hipSuccess) e = hipMalloc(&s->cellBody, cells * sizeof(int32_t));
@@ -1004,6 +1145,9 @@ fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distm
 
 fc_status fc_body_set_destroy(fc_body_set* s) {
   if (!s) return FC_SUCCESS;
+  hipFree(s->netOf);
+  hipFree(s->order);
+  hipFree(s->bins);
   hipFree(s->bodies);
   hipFree(s->cellBody);
   hipFree(s->margins);
@@ -1024,28 +1168,60 @@ fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrog
   hipStream_t st = ctx->stream;
   const LatticeArg L{set->lattice[0], set->lattice[1], set->lattice[2], set->latticeOrigin[0],
                      set->latticeOrigin[1], set->latticeOrigin[2], set->latticeCell};
+  StageTimer* timer = ctx->timer.enabled ? &ctx->timer : nullptr;
   if (surrogate) {
-    if (surrogate->B != set->B)
-      return set_error(FC_ERROR_INVALID_VALUE, "surrogate and body set differ in body count");
+    if ((uint32_t)set->maxNet >= surrogate->B)
+      return set_error(FC_ERROR_INVALID_VALUE, "a body names a network the surrogate lacks");
     if (!candidate) return set_error(FC_ERROR_INVALID_VALUE, "the surrogate path needs candidate");
     std::vector<float> m(set->B, 0.0f);
     if (margins) std::memcpy(m.data(), margins, set->B * sizeof(float));
     FC_HIP_TRY(hipMemcpyAsync(set->margins, m.data(), set->B * sizeof(float),
                               hipMemcpyHostToDevice, st));
     FC_TRY(sdf_pack(surrogate));
-    const uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = (tiles + 3) / 4;
-    if (blocks > 4096u) blocks = 4096u;
-    hipLaunchKernelGGL(k_boundary_cull, dim3(blocks), dim3(kTB), 0, st, surrogate->packed,
-                       surrogate->params, surrogate->freq, surrogate->frame, set->bodies,
-                       set->cellBody, L, set->B, set->margins, (const float4*)pos, n, radius,
-                       candidate);
+    const uint32_t nets = (uint32_t)set->maxNet + 1;
+    if (nets > 4095u) return set_error(FC_ERROR_CAPACITY, "at most 4095 networks");
+    if (set->capacity < n) {
+      FC_HIP_TRY(hipStreamSynchronize(st));
+      hipFree(set->netOf);
+      hipFree(set->order);
+      set->netOf = nullptr;
+      set->order = nullptr;
+      set->capacity = 0;
+      FC_HIP_TRY(hipMalloc(&set->netOf, (size_t)n * sizeof(uint16_t)));
+      FC_HIP_TRY(hipMalloc(&set->order, (size_t)n * sizeof(uint32_t)));
+      set->capacity = n;
+    }
+    if (!set->bins) FC_HIP_TRY(hipMalloc(&set->bins, 2 * 4098 * sizeof(uint32_t)));
+    uint32_t* cnt = set->bins;
+    uint32_t* cursor = set->bins + 4098;
+    if (counts) FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
+    if (timer) timer->begin(st, "boundary_bucket");
+    FC_HIP_TRY(launch_fill_u32(st, cnt, 0u, nets + 1));
+    const unsigned blocks = (n + kBucketChunk - 1) / kBucketChunk;
+    hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(kTB), (nets + 1) * sizeof(uint32_t), st,
+                       set->bodies, set->cellBody, L, set->B, (const float4*)pos, n, nets,
+                       set->netOf, cnt);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kTB), 0, st, cnt, nets, cursor);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks), dim3(kTB), 2 * (nets + 1) * sizeof(uint32_t),
+                       st, set->netOf, n, nets, cursor, set->order);
+    if (timer) timer->end(st);
     FC_HIP_TRY(hipGetLastError());
+    const uint32_t cblocks = mfma_blocks(n);
+    if (timer) timer->begin(st, "boundary_cull");
+    hipLaunchKernelGGL(k_boundary_cull, dim3(cblocks), dim3(kTB), 0, st, surrogate->packed,
+                       surrogate->params, surrogate->freq, surrogate->frame, set->bodies,
+                       set->cellBody, L, set->B, set->margins, (const float4*)pos, set->order, n,
+                       radius, candidate);
+    if (timer) timer->end(st);
+    FC_HIP_TRY(hipGetLastError());
+  } else if (counts) {
+    FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
   }
-  if (counts) FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
+  if (timer) timer->begin(st, surrogate ? "boundary_exact_band" : "boundary_exact");
   hipLaunchKernelGGL(k_boundary_exact, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, st, set->bodies,
                      set->cellBody, L, set->B, (const float4*)pos, n, radius,
                      surrogate ? candidate : nullptr, contact, counts ? set->counters : nullptr);
+  if (timer) timer->end(st);
   FC_HIP_TRY(hipGetLastError());
   if (counts) {
     FC_HIP_TRY(hipMemcpyAsync(counts, set->counters, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -430,24 +430,28 @@ fc_status fc_sdf_eval(fc_sdf* sdf, const uint32_t* bodies, const float* points, 
 
 /* Rigid bodies of a boundary.  Body b has a distance map in its model frame
  * and a pose: model2world (row-major rotation) and the world position of
- * its model origin.  A coarse lattice (lattice[3] cells of size latticeCell
- * from latticeOrigin, cellBody[(z*ly + y)*lx + x] = body or -1) assigns at
- * most one body to each region; a sphere is tested against the body of its
- * lattice cell.  The maps must outlive the set. */
+ * its model origin; bodyNet[b] (NULL: b) is the surrogate network of its
+ * shape (bodies of one shape may share a map and a network).  A coarse
+ * lattice (lattice[3] cells of size latticeCell from latticeOrigin,
+ * cellBody[(z*ly + y)*lx + x] = body or -1) assigns at most one body to
+ * each region; a sphere is tested against the body of its lattice cell.
+ * The maps must outlive the set. */
 typedef struct fc_body_set fc_body_set;
 fc_status fc_body_set_create(fc_context* ctx, uint32_t numBodies, const fc_distmap* const* maps,
                              const float* model2world, const float* origins,
-                             const int32_t lattice[3], const float latticeOrigin[3],
-                             float latticeCell, const int32_t* cellBody, fc_body_set** out);
+                             const int32_t* bodyNet, const int32_t lattice[3],
+                             const float latticeOrigin[3], float latticeCell,
+                             const int32_t* cellBody, fc_body_set** out);
 fc_status fc_body_set_destroy(fc_body_set* set);
 /* Contacts of n spheres (device float4 positions, common radius) with the
  * bodies: contact[i] = 1 when sphere i lies in its body's map box and the
  * map distance (queryMap of the model-frame centre) is < radius, the test of
  * checkDistMap (boundaries.cuh:38-89) per body.  With surrogate == NULL
- * every sphere in a box gets the exact query.  With a surrogate (one
- * network per body, same body order) the BF16 MFMA network first writes
- * candidate[i] = 1 for spheres whose predicted distance is <= radius +
- * margins[body] (host array, NULL = 0), and only those get the exact query.
+ * every sphere in a box gets the exact query.  With a surrogate (network
+ * bodyNet[b] for body b) the spheres are grouped by network, the BF16 MFMA
+ * network predicts every in-box sphere's distance, candidate[i] = 1 marks
+ * those within radius + margins[body] (host array, NULL = 0), and only the
+ * candidates get the exact query.
  * counts (host, may be NULL; synchronises): spheres in a box, exact queries,
  * contacts. */
 fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrogate,

@@ -26,16 +26,16 @@ def _threads():
     orc.set_threads(1)
 
 
-@pytest.fixture(scope="module")
-def scene(gpu_ctx):
+@pytest.fixture(scope="module", params=[None, 3], ids=["shape-per-body", "3-shapes"])
+def scene(gpu_ctx, request):
     from fullc0ntact_amd.boundary import build_scene
-    print("building an 8-body scene", flush=True)
+    print(f"building an 8-body scene, shapes={request.param}", flush=True)
     sc = build_scene(gpu_ctx, lattice=2, cells=40, samples=1 << 15, epochs=40, seed=11,
-                     radius=1.0 / 64)
+                     radius=1.0 / 64, shapes=request.param)
     yield sc
     sc.bodies.close()
     sc.surrogate.close()
-    for m in sc.maps:
+    for m in sc.shape_maps:
         m.close()
 
 
