@@ -122,12 +122,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
 // reorderDataAndFindCellStartD (particles_kernel.cu:119-182).  The
 // neighbour's hash comes from the previous lane through a cross-lane
 // shuffle; lane 0 of each wave reads it from memory.
+// With slotOf, particle id p's state lies at slot slotOf[p] of oldPos/oldVel
+// (the sorted-slot layout kept between the fused steps of a run).
 __global__ __launch_bounds__(kBlock) void k_reorder(
     uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd,
     float4* __restrict__ sortedPos, float4* __restrict__ sortedVel,
     const uint32_t* __restrict__ hash, const uint32_t* __restrict__ index,
-    const float4* __restrict__ oldPos, const float4* __restrict__ oldVel, uint32_t n,
-    uint32_t numCells) {
+    const uint32_t* __restrict__ slotOf, const float4* __restrict__ oldPos,
+    const float4* __restrict__ oldVel, uint32_t n, uint32_t numCells) {
   uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   uint32_t h = (k < n) ? hash[k] : 0u;
   uint32_t hprev = __shfl_up(h, 1);
@@ -141,6 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
   }
   if (k == n - 1 && h < numCells) cellEnd[h] = k + 1;
   uint32_t src = index[k];
+  if (slotOf) src = slotOf[src];
   sortedPos[k] = oldPos[src];
   sortedVel[k] = oldVel[src];
 }
@@ -221,13 +224,17 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
   newVel[orig] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
-// The collide stage.  newVel[index[k]] = (vel + force, 0).
+// The collide stage.  newVel[index[k]] = (vel + force, 0), and with posOut
+// also posOut[index[k]] = sortedPos[k] (the last step of a multi-step run
+// returns the positions to id order).
 // With kFuse the kernel also performs the next update's first stage for the
-// same particle, which the update order would run next on the same arrays:
-// integrate (from the collided velocity and the unchanged position) into
-// pos/newVel, the cell hash into keysOut, and the radix sort's digit
-// histogram of those keys.  Every value written is the one the separate
-// stages would write.
+// same particle, which the update order would run next: integrate (from the
+// collided velocity and the unchanged position), the cell hash and the
+// radix sort's digit histogram of the hashes.  The integrated state is
+// written at the sorted slot k (pos[k], newVel[k], coalesced), the hash at
+// the particle's id (keysOut[id], so that the stable sort still orders ties
+// by id) and slotOf[id] = k for the next reorder.  Every value is the one
+// the separate stages would compute.
 // kPoly takes each sphere's radius from pos.w (a radius above
 // particleRadius raises DEVERR_RADIUS: the neighbour search and the contact
 // pre-filter are sized by particleRadius).
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-    uint32_t* devFlags) {
+    uint32_t* devFlags, uint32_t* __restrict__ slotOf) {
   __shared__ uint32_t s_list[kListCap][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -257,13 +264,15 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t orig = index[k];
     if (!kFuse) {
       newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+      if (pos) pos[orig] = p4;
     } else {
       v3f p = xyz(p4), v = nv;
       integrate_with(P, B, p4.w, dt, p, v);
-      pos[orig] = make_float4(p.x, p.y, p.z, p4.w);
-      newVel[orig] = make_float4(v.x, v.y, v.z, 0.0f);
+      pos[k] = make_float4(p.x, p.y, p.z, p4.w);
+      newVel[k] = make_float4(v.x, v.y, v.z, 0.0f);
       key = hash_of(P, p.x, p.y, p.z);
       keysOut[orig] = key;
+      slotOf[orig] = k;
     }
   }
   if (kFuse) {
@@ -327,10 +336,10 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const Bo
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
-                          uint32_t numCells) {
+                          uint32_t numCells, const uint32_t* slotOf) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, cellStart, cellEnd,
-                     sortedPos, sortedVel, hash, index, oldPos, oldVel, n, numCells);
+                     sortedPos, sortedVel, hash, index, slotOf, oldPos, oldVel, n, numCells);
   return hipGetLastError();
 }
 
@@ -349,23 +358,25 @@ float contact_d2_threshold(float R) {
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
-                          const uint32_t* cellEnd, uint32_t n) {
+                          const uint32_t* cellEnd, uint32_t n, float4* posOut) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   // The contact-list kernel packs a sorted index into 27 bits and assumes
   // the three cells of a neighbour row are distinct (G.x >= 4).
-  if (n >= (1u << 27) || P.gridSize.x < 4)
+  if (n >= (1u << 27) || P.gridSize.x < 4) {
+    if (posOut) return hipErrorInvalidValue;  // only the multi-step run asks for positions
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
                        cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
-  else if (poly)
+  } else if (poly) {
     hipLaunchKernelGGL((k_collide_v3<false, true>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
-                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr, devFlags);
-  else
+                       posOut, 0.0f, nullptr, 0, nullptr, nullptr, devFlags, nullptr);
+  } else {
     hipLaunchKernelGGL((k_collide_v3<false, false>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
-                       nullptr, 0.0f, nullptr, 0, nullptr, nullptr, devFlags);
+                       posOut, 0.0f, nullptr, 0, nullptr, nullptr, devFlags, nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -378,18 +389,19 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const float4* sortedPos, const float4* sortedVel,
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
-                                SortScratch& sort, int passes, uint32_t* devFlags) {
+                                SortScratch& sort, int passes, uint32_t* devFlags,
+                                uint32_t* slotOf) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
   if (B.poly)
     hipLaunchKernelGGL((k_collide_v3<true, true>), g, b, 0, s, P, B, d2, vel, sortedPos, sortedVel,
                        index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut, passes,
-                       sort.histogram, sort.counters, devFlags);
+                       sort.histogram, sort.counters, devFlags, slotOf);
   else
     hipLaunchKernelGGL((k_collide_v3<true, false>), g, b, 0, s, P, B, d2, vel, sortedPos,
                        sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt,
-                       keysOut, passes, sort.histogram, sort.counters, devFlags);
+                       keysOut, passes, sort.histogram, sort.counters, devFlags, slotOf);
   return hipGetLastError();
 }
 

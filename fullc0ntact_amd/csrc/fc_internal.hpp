@@ -149,15 +149,17 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const Bo
 hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
                                     float4* pos, float4* vel, float dt, uint32_t n);
 DistMapView distmap_view(const fc_distmap* m);
+// slotOf (optional): particle id p's state is at slot slotOf[p] of oldPos/oldVel.
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
-                          uint32_t numCells);
+                          uint32_t numCells, const uint32_t* slotOf = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
+// posOut (optional): also scatter the sorted positions to id order.
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
-                          const uint32_t* cellEnd, uint32_t n);
+                          const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr);
 
 float contact_d2_threshold(float R);
 // collide + the next update's integrate/hash/histogram in one launch (see
@@ -168,7 +170,8 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const float4* sortedPos, const float4* sortedVel,
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
-                                SortScratch& sort, int passes, uint32_t* devFlags);
+                                SortScratch& sort, int passes, uint32_t* devFlags,
+                                uint32_t* slotOf);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).
@@ -207,6 +210,7 @@ struct fc_system {
   uint32_t* hash = nullptr;       // sorted hashes of the last update
   uint32_t* index = nullptr;      // sorted original indices of the last update
   uint32_t* keys = nullptr;       // unsorted hashes written by integrate+hash
+  uint32_t* slotOf = nullptr;     // multi-step runs: sorted slot of each particle id
   uint32_t* cellStart = nullptr;
   uint32_t* cellEnd = nullptr;
   // True when the cell tables hold exactly the cells listed in `hash`, so the

@@ -42,10 +42,11 @@ void free_system_buffers(fc_system* sys) {
   hipFree(sys->hash);
   hipFree(sys->index);
   hipFree(sys->keys);
+  hipFree(sys->slotOf);
   hipFree(sys->cellStart);
   hipFree(sys->cellEnd);
   sys->pos = sys->vel = sys->sortedPos = sys->sortedVel = nullptr;
-  sys->hash = sys->index = sys->keys = sys->cellStart = sys->cellEnd = nullptr;
+  sys->hash = sys->index = sys->keys = sys->slotOf = sys->cellStart = sys->cellEnd = nullptr;
 }
 
 void drop_graph(fc_system* sys) {
@@ -120,6 +121,12 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
 //   middle: sort(+clear) reorder [collide+next integrate]    x (steps - 2)
 //   tail:   sort(+clear) reorder [collide]
 // which computes exactly what `steps` separate updates compute.
+// Between the fused steps the particle state stays in the previous sort's
+// order: the fused collide writes pos/vel at each particle's sorted slot
+// (coalesced) and records the slot in slotOf[id]; only the hash is written
+// at the id, so the sort input and its tie order are those of the separate
+// updates.  The next reorder gathers through slotOf, and the tail's collide
+// scatters the positions (with the velocities) back to id order.
 int run_passes(const fc_sim_params& P) {
   const int p = radix_passes(key_bits_for_cells(P.numCells), false);
   return p < 2 ? 2 : p;
@@ -153,7 +160,9 @@ fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
 
 // sort (clearing the previous cells in its first pass unless the head just
 // did) + reorder + collide, fused with the next integrate unless `last`.
-fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last,
+// fromSlots: pos/vel hold the sorted-slot layout of the previous fused
+// collide (every body but the first after the head).
+fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last, bool fromSlots,
                            StageTimer* timer) {
   fc_context* ctx = sys->ctx;
   hipStream_t st = ctx->stream;
@@ -167,18 +176,19 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   if (e != hipSuccess) return hip_error(e, "sort");
   if (timer) timer->begin(st, "reorder");
   e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,
-                     sys->index, sys->pos, sys->vel, n, P.numCells);
+                     sys->index, sys->pos, sys->vel, n, P.numCells,
+                     fromSlots ? sys->slotOf : nullptr);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "reorder");
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
-                       sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n);
+                       sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n,
+                       fromSlots ? sys->pos : nullptr);
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
-                             sys->sortedVel,
-                             sys->index, sys->cellStart, sys->cellEnd, n, dt, sys->keys,
-                             ctx->sort, passes, ctx->devFlags);
+                             sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
+                             sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
@@ -188,7 +198,8 @@ fc_status capture_graph(fc_system* sys, float dt) {
   drop_graph(sys);
   hipStream_t st = sys->ctx->stream;
   FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  fc_status s = enqueue_run_body(sys, dt, /*clearInSort=*/true, /*last=*/false, nullptr);
+  fc_status s = enqueue_run_body(sys, dt, /*clearInSort=*/true, /*last=*/false,
+                                 /*fromSlots=*/true, nullptr);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (s != FC_SUCCESS) {
@@ -296,6 +307,7 @@ fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 grid
   if (e == hipSuccess) e = hipMalloc(&sys->hash, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->index, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&sys->slotOf, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemset(sys->pos, 0, n * sizeof(float4));
   if (e == hipSuccess) e = hipMemset(sys->vel, 0, n * sizeof(float4));
   if (e != hipSuccess) {
@@ -404,12 +416,13 @@ fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
   }
   StageTimer* timer = ctx->timer.enabled ? &ctx->timer : nullptr;
   FC_TRY(enqueue_run_head(sys, deltaTime, timer));
-  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/false, /*last=*/false, timer));
+  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/false, /*last=*/false,
+                          /*fromSlots=*/false, timer));
   const uint32_t middle = steps - 2;
   if (middle > 0) {
     if (timer) {
       for (uint32_t i = 0; i < middle; ++i)
-        FC_TRY(enqueue_run_body(sys, deltaTime, true, false, timer));
+        FC_TRY(enqueue_run_body(sys, deltaTime, true, false, true, timer));
     } else {
       if (!sys->graphExec || sys->graphDt != deltaTime ||
           std::memcmp(&sys->graphParams, &sys->params, sizeof(fc_sim_params)) != 0) {
@@ -419,7 +432,8 @@ fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
         FC_HIP_TRY(hipGraphLaunch(sys->graphExec, ctx->stream));
     }
   }
-  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/true, /*last=*/true, timer));
+  FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/true, /*last=*/true,
+                          /*fromSlots=*/true, timer));
   return FC_SUCCESS;
 }
 
