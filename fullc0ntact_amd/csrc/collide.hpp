@@ -42,7 +42,25 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t b, uint32_t nb, u
 constexpr uint32_t kCollideXcdGroup = FC_COLLIDE_XCD_GROUP;
 
 // Capacity of the per-lane contact list of k_collide_v3.
-constexpr int kListCap = 28;
+#ifndef FC_LIST_CAP
+#define FC_LIST_CAP 28
+#endif
+constexpr int kListCap = FC_LIST_CAP;
+// Candidates per iteration of collide_one_rows' phase A and contacts per
+// iteration of its phase B (loads issued together).  The kernel is bound by
+// the latency of its dependent neighbour loads more than by issue: on the
+// settled 1M-sphere bed, batches of (2, 1) measured 136 us per launch,
+// (4, 2) 122 us and (4, 4) 113 us; (8, 4) needs 97 VGPRs and gained
+// nothing.  Raising occupancy instead (list capacity 20 or 16 with 6-8
+// waves per SIMD) spilled registers and was slower.
+#ifndef FC_ROWS_BATCH
+#define FC_ROWS_BATCH 4
+#endif
+#ifndef FC_PAIR_BATCH
+#define FC_PAIR_BATCH 4
+#endif
+constexpr int kRowsBatch = FC_ROWS_BATCH;
+constexpr int kPairBatch = FC_PAIR_BATCH;
 // Candidates whose loads are issued together in k_collide_v3.
 constexpr int kBatch = 4;
 
@@ -173,6 +191,177 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
             if (j != k) {
               const float4 pj = sortedPos[j];
               v3f f = collide_spheres(pos, xyz(pj), vel, xyz(sortedVel[j]), rA,
+                                      kPoly ? pj.w : rA, P.spring, P.damping, P.shear,
+                                      P.attraction);
+              cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+            }
+          force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+        }
+  }
+  v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
+  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), rA, P.colliderRadius,
+                          P.spring, P.damping, P.shear, 0.0f);
+  force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
+  return make_v3(vel.x + force.x, vel.y + force.y, vel.z + force.z);
+}
+
+// Load through a 32-bit byte offset from a uniform base pointer, so that the
+// address is the base (scalar registers) plus one 32-bit offset register.
+template <typename T>
+__device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteOff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byteOff);
+}
+
+// Row-range variant of collide_one_v3 for grids with every dimension >= 4,
+// numCells <= 2^30 and the sorted hashes at hand.
+//
+// The three cells of a neighbour row (dx = -1, 0, 1) have consecutive
+// hashes unless the row wraps in x, so their particles are one contiguous
+// range [lo, lo + len) of the sorted order, visited in the cell order of
+// the 27-cell loop.  (A wrapping row is still one range when one of its two
+// hash runs is empty; a lane with a row of two non-empty runs takes the
+// direct loop.)
+//
+// Phase A walks the three rows of each z-plane as one flattened sequence
+// and keeps the indices of candidates that pass a pre-filter in a per-lane
+// list in LDS.  The pre-filter evaluates d2 with fused multiply-adds and
+// compares against contactD2 raised by 2^-18 relative, which keeps every
+// candidate with (reference-rounded) d2 < contactD2; the extra near-misses
+// get the exact contact test in phase B and add +0.  Phase B walks the list
+// and detects a change of cell by a change of the neighbour's sorted hash
+// (all 27 cells are distinct when every grid dimension is >= 3), adding
+// each cell's partial to the total in cell order.  As in collide_one_v3
+// every dropped or non-touching candidate contributed exactly +0, so the
+// result is bit-identical to the 27-cell loop.
+template <bool kPoly, int BS>
+__device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invCell,
+                                                float contactD2, uint32_t k, v3f pos, v3f vel,
+                                                float rA, const float4* __restrict__ sortedPos,
+                                                const float4* __restrict__ sortedVel,
+                                                const uint32_t* __restrict__ sortedHash,
+                                                const uint32_t* __restrict__ cellStart,
+                                                const uint32_t* __restrict__ cellEnd, uint32_t n,
+                                                uint32_t (*s_list)[BS]) {
+  const v3f q = cell_quotients(P, invCell, pos);
+  const int gx = cell_of_quotient(q.x), gy = cell_of_quotient(q.y), gz = cell_of_quotient(q.z);
+  const uint32_t Gx = P.gridSize.x, Gy = P.gridSize.y;
+  const uint32_t mx = Gx - 1u, my = Gy - 1u, mz = P.gridSize.z - 1u;
+  const uint32_t xc = (uint32_t)gx & mx;
+  const uint32_t x0 = ((uint32_t)gx - 1u) & mx, x2 = ((uint32_t)gx + 1u) & mx;
+
+  // ---- row ranges ----
+  uint32_t s[27], e[27];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const uint32_t yy = ((uint32_t)gy + (uint32_t)(r % 3) - 1u) & my;
+    const uint32_t zz = ((uint32_t)gz + (uint32_t)(r / 3) - 1u) & mz;
+    const uint32_t rowBase = (zz * Gy + yy) * Gx;
+    const uint32_t o0 = (rowBase + x0) << 2, o1 = (rowBase + xc) << 2, o2 = (rowBase + x2) << 2;
+    s[3 * r] = load_off(cellStart, o0);
+    s[3 * r + 1] = load_off(cellStart, o1);
+    s[3 * r + 2] = load_off(cellStart, o2);
+    e[3 * r] = load_off(cellEnd, o0);
+    e[3 * r + 1] = load_off(cellEnd, o1);
+    e[3 * r + 2] = load_off(cellEnd, o2);
+  }
+  const bool wrapLo = xc == 0u, wrapHi = xc == mx;
+  uint32_t lo[9], len[9];
+  bool split = false;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const bool n0 = s[3 * r] != CELL_EMPTY;
+    const bool n1 = s[3 * r + 1] != CELL_EMPTY;
+    const bool n2 = s[3 * r + 2] != CELL_EMPTY;
+    const uint32_t e0 = e[3 * r] > n ? n : e[3 * r];
+    const uint32_t e1 = e[3 * r + 1] > n ? n : e[3 * r + 1];
+    const uint32_t e2 = e[3 * r + 2] > n ? n : e[3 * r + 2];
+    const uint32_t a = n0 ? s[3 * r] : (n1 ? s[3 * r + 1] : s[3 * r + 2]);
+    const uint32_t b = n2 ? e2 : (n1 ? e1 : e0);
+    lo[r] = a;
+    len[r] = (n0 || n1 || n2) && b > a ? b - a : 0u;
+    split = split || (wrapLo && n0 && (n1 || n2)) || (wrapHi && n2 && (n0 || n1));
+  }
+
+  // ---- phase A: candidates within the contact pre-filter ----
+  const float thrA = fmaxf(contactD2 * (1.0f + 0x1p-18f), contactD2 + 0x1p-140f);
+  const uint32_t last = n - 1u;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int pl = 0; pl < 3; ++pl) {
+    const uint32_t b1 = len[3 * pl], b2 = b1 + len[3 * pl + 1], tot = b2 + len[3 * pl + 2];
+    const uint32_t q0 = lo[3 * pl], q1 = lo[3 * pl + 1] - b1, q2 = lo[3 * pl + 2] - b2;
+    // kRowsBatch candidates per iteration; the index of a slot past the
+    // plane's last candidate is clamped into the arrays and its test is
+    // masked off, so the loads of one iteration are all in flight at once.
+#pragma unroll 1
+    for (uint32_t t = 0; t < tot; t += kRowsBatch) {
+      uint32_t jj[kRowsBatch];
+      float4 qq[kRowsBatch];
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u) {
+        const uint32_t tu = t + (uint32_t)u;
+        const uint32_t j = tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2));
+        jj[u] = j < last ? j : last;
+        qq[u] = load_off(sortedPos, jj[u] << 4);
+      }
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u) {
+        const float ex = qq[u].x - pos.x, ey = qq[u].y - pos.y, ez = qq[u].z - pos.z;
+        const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = jj[u];
+        cnt += (d2 < thrA && jj[u] != k && (u == 0 || t + (uint32_t)u < tot)) ? 1u : 0u;
+      }
+    }
+  }
+
+  v3f force = make_v3(0.0f, 0.0f, 0.0f);
+  if (!split && cnt <= (uint32_t)kListCap) {
+    // ---- phase B ----
+    // kPairBatch list entries per iteration (entries past cnt load a
+    // clamped index and are masked off).
+    v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+    uint32_t curHash = 0xffffffffu;  // no cell has this hash (numCells <= 2^30)
+#pragma unroll 1
+    for (uint32_t i = 0; i < cnt; i += kPairBatch) {
+      float4 qp[kPairBatch], qv[kPairBatch];
+      uint32_t hh[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const uint32_t iu = i + (uint32_t)u;
+        uint32_t j = s_list[iu < (uint32_t)kListCap ? iu : (uint32_t)kListCap][threadIdx.x];
+        j = (u == 0 || iu < cnt) && j < last ? j : last;
+        qp[u] = load_off(sortedPos, j << 4);
+        qv[u] = load_off(sortedVel, j << 4);
+        hh[u] = load_off(sortedHash, j << 2);
+      }
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        if (u == 0 || i + (uint32_t)u < cnt) {
+          if (hh[u] != curHash) {
+            force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+            cf = make_v3(0.0f, 0.0f, 0.0f);
+            curHash = hh[u];
+          }
+          v3f f = collide_spheres(pos, xyz(qp[u]), vel, xyz(qv[u]), rA, kPoly ? qp[u].w : rA,
+                                  P.spring, P.damping, P.shear, P.attraction);
+          cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+        }
+      }
+    }
+    force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+  } else {
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const uint32_t h = grid_hash(gx + dx, gy + dy, gz + dz, P.gridSize);
+          const uint32_t start = cellStart[h];
+          if (start == CELL_EMPTY) continue;
+          const uint32_t end = cellEnd[h] > n ? n : cellEnd[h];
+          v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+          for (uint32_t jj = start; jj < end; ++jj)
+            if (jj != k) {
+              const float4 pj = sortedPos[jj];
+              v3f f = collide_spheres(pos, xyz(pj), vel, xyz(sortedVel[jj]), rA,
                                       kPoly ? pj.w : rA, P.spring, P.damping, P.shear,
                                       P.attraction);
               cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);

@@ -12,6 +12,7 @@
 #include "collide.hpp"
 
 #include <cmath>
+#include <cstring>
 
 namespace fc {
 
@@ -238,15 +239,18 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
 // kPoly takes each sphere's radius from pos.w (a radius above
 // particleRadius raises DEVERR_RADIUS: the neighbour search and the contact
 // pre-filter are sized by particleRadius).
-template <bool kFuse, bool kPoly>
+// kRows selects the row-range neighbour walk (collide_one_rows), which
+// needs the sorted hashes.
+template <bool kFuse, bool kPoly, bool kRows>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
     fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
     const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
     const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-    uint32_t* devFlags, uint32_t* __restrict__ slotOf) {
-  __shared__ uint32_t s_list[kListCap][kBlock];
+    uint32_t* devFlags, uint32_t* __restrict__ slotOf, const uint32_t* __restrict__ sortedHash,
+    v3f invCell) {
+  __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
   const bool valid = k < n;
@@ -259,8 +263,12 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const float4 p4 = sortedPos[k];
     const float rA = kPoly ? p4.w : P.particleRadius;
     if (kPoly && !(rA <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
-    const v3f nv = collide_one_v3<kPoly, kBlock>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
-                                         sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
+    const v3f nv =
+        kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, xyz(p4),
+                                                xyz(sortedVel[k]), rA, sortedPos, sortedVel,
+                                                sortedHash, cellStart, cellEnd, n, s_list)
+              : collide_one_v3<kPoly, kBlock>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
+                                              sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
     const uint32_t orig = index[k];
     if (!kFuse) {
       newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
@@ -270,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
       integrate_with(P, B, p4.w, dt, p, v);
       pos[k] = make_float4(p.x, p.y, p.z, p4.w);
       newVel[k] = make_float4(v.x, v.y, v.z, 0.0f);
-      key = hash_of(P, p.x, p.y, p.z);
+      key = hash_of_q(P, cell_quotients(P, invCell, p));
       keysOut[orig] = key;
       slotOf[orig] = k;
     }
@@ -355,10 +363,54 @@ float contact_d2_threshold(float R) {
   return t;
 }
 
+// The row-range walk needs the sorted hashes, 27 distinct neighbour cells
+// (every grid dimension >= 4 here) and cell byte offsets below 2^32.
+static bool rows_walk_ok(const fc_sim_params& P, const uint32_t* sortedHash) {
+  return sortedHash != nullptr && P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
+         P.numCells <= (1u << 30);
+}
+
+// Exact reciprocal of c when c is a positive normal power of two whose
+// reciprocal is normal too, else 0.
+static float pow2_reciprocal(float c) {
+  uint32_t b;
+  std::memcpy(&b, &c, 4);
+  const uint32_t ex = (b >> 23) & 0xffu;
+  return ((b & 0x807fffffu) == 0u && ex >= 1u && ex <= 253u) ? 1.0f / c : 0.0f;
+}
+
+// Reciprocal cell sizes for cell_quotients: all three or none.
+static v3f inv_cell(const fc_sim_params& P) {
+  const float ix = pow2_reciprocal(P.cellSize.x), iy = pow2_reciprocal(P.cellSize.y),
+              iz = pow2_reciprocal(P.cellSize.z);
+  if (ix == 0.0f || iy == 0.0f || iz == 0.0f) return v3f{0.0f, 0.0f, 0.0f};
+  return v3f{ix, iy, iz};
+}
+
+template <bool kFuse, bool kPoly>
+static void launch_v3(hipStream_t s, bool rows, dim3 g, dim3 b, const fc_sim_params& P,
+                      const BoundaryArg& B, float d2, float4* newVel, const float4* sortedPos,
+                      const float4* sortedVel, const uint32_t* index, const uint32_t* cellStart,
+                      const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
+                      uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
+                      uint32_t* devFlags, uint32_t* slotOf, const uint32_t* sortedHash) {
+  if (rows)
+    hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
+                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
+                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, sortedHash,
+                       inv_cell(P));
+  else
+    hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
+                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
+                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, sortedHash,
+                       inv_cell(P));
+}
+
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
-                          const uint32_t* cellEnd, uint32_t n, float4* posOut) {
+                          const uint32_t* cellEnd, uint32_t n, float4* posOut,
+                          const uint32_t* sortedHash) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
@@ -368,15 +420,17 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
     if (posOut) return hipErrorInvalidValue;  // only the multi-step run asks for positions
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
                        cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
-  } else if (poly) {
-    hipLaunchKernelGGL((k_collide_v3<false, true>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
-                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
-                       posOut, 0.0f, nullptr, 0, nullptr, nullptr, devFlags, nullptr);
-  } else {
-    hipLaunchKernelGGL((k_collide_v3<false, false>), g, b, 0, s, P, BoundaryArg{}, d2, newVel,
-                       sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup,
-                       posOut, 0.0f, nullptr, 0, nullptr, nullptr, devFlags, nullptr);
+    return hipGetLastError();
   }
+  const bool rows = rows_walk_ok(P, sortedHash);
+  if (poly)
+    launch_v3<false, true>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
+                           index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
+                           nullptr, devFlags, nullptr, sortedHash);
+  else
+    launch_v3<false, false>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
+                            index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
+                            nullptr, devFlags, nullptr, sortedHash);
   return hipGetLastError();
 }
 
@@ -390,18 +444,19 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf) {
+                                uint32_t* slotOf, const uint32_t* sortedHash) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
+  const bool rows = rows_walk_ok(P, sortedHash);
   if (B.poly)
-    hipLaunchKernelGGL((k_collide_v3<true, true>), g, b, 0, s, P, B, d2, vel, sortedPos, sortedVel,
-                       index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt, keysOut, passes,
-                       sort.histogram, sort.counters, devFlags, slotOf);
+    launch_v3<true, true>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
+                          cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
+                          devFlags, slotOf, sortedHash);
   else
-    hipLaunchKernelGGL((k_collide_v3<true, false>), g, b, 0, s, P, B, d2, vel, sortedPos,
-                       sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos, dt,
-                       keysOut, passes, sort.histogram, sort.counters, devFlags, slotOf);
+    launch_v3<true, false>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
+                           cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
+                           devFlags, slotOf, sortedHash);
   return hipGetLastError();
 }
 

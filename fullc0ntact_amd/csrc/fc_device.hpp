@@ -59,6 +59,31 @@ __device__ __forceinline__ uint32_t hash_of(const fc_sim_params& P, float px, fl
   return grid_hash(gx, gy, gz, P.gridSize);
 }
 
+// cell_coord split into its quotient and the saturating floor, for kernels
+// that also use the quotient.
+__device__ __forceinline__ int cell_of_quotient(float q) {
+  q = floorf(q);
+  if (!(q == q)) return 0;
+  if (q >= 2147483648.0f) return 2147483647;
+  if (q < -2147483648.0f) return (int)0x80000000;
+  return (int)q;
+}
+
+// (p - origin) / cellSize per axis.  invCell holds the exact reciprocals of
+// the cell sizes when all three are powers of two (then x / c and x * (1/c)
+// are the same real number and round identically, so the multiply is the
+// division), else zeros; the choice is uniform across the launch.
+__device__ __forceinline__ v3f cell_quotients(const fc_sim_params& P, v3f invCell, v3f p) {
+  const v3f d = make_v3(p.x - P.worldOrigin.x, p.y - P.worldOrigin.y, p.z - P.worldOrigin.z);
+  if (invCell.x != 0.0f) return make_v3(d.x * invCell.x, d.y * invCell.y, d.z * invCell.z);
+  return make_v3(d.x / P.cellSize.x, d.y / P.cellSize.y, d.z / P.cellSize.z);
+}
+
+__device__ __forceinline__ uint32_t hash_of_q(const fc_sim_params& P, v3f q) {
+  return grid_hash(cell_of_quotient(q.x), cell_of_quotient(q.y), cell_of_quotient(q.z),
+                   P.gridSize);
+}
+
 // Pair force of sphere B on sphere A (collideSpheres,
 // particles_kernel.cu:185-218):
 //   d = pB - pA; dist = sqrt(d.d); R = rA + rB
@@ -83,7 +108,12 @@ __device__ __forceinline__ v3f collide_spheres(v3f pA, v3f pB, v3f vA, v3f vB, f
     f = make_v3(nrm.x * s, nrm.y * s, nrm.z * s);
     f = make_v3(f.x + rv.x * damping, f.y + rv.y * damping, f.z + rv.z * damping);
     f = make_v3(f.x + tv.x * shear, f.y + tv.y * shear, f.z + tv.z * shear);
-    f = make_v3(f.x + rel.x * attraction, f.y + rel.y * attraction, f.z + rel.z * attraction);
+    // With attraction == 0 the term is +-0 (rel is finite here), which can
+    // only turn a -0 component into +0.  Every caller adds the result to a
+    // sum that starts at +0 and therefore never holds -0, where x + (-0)
+    // and x + (+0) are equal, so the term is skipped exactly.
+    if (attraction != 0.0f)
+      f = make_v3(f.x + rel.x * attraction, f.y + rel.y * attraction, f.z + rel.z * attraction);
   }
   return f;
 }

@@ -156,10 +156,13 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
                           uint32_t numCells, const uint32_t* slotOf = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
 // posOut (optional): also scatter the sorted positions to id order.
+// sortedHash (optional): the sorted hashes, which enable the row-range
+// neighbour walk.
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
-                          const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr);
+                          const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr,
+                          const uint32_t* sortedHash = nullptr);
 
 float contact_d2_threshold(float R);
 // collide + the next update's integrate/hash/histogram in one launch (see
@@ -171,7 +174,7 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf);
+                                uint32_t* slotOf, const uint32_t* sortedHash);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).

@@ -106,7 +106,8 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   if (e != hipSuccess) return hip_error(e, "reorder");
   if (timer) timer->begin(st, "collide");
   e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
-                     sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n);
+                     sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr,
+                     sys->hash);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   sys->tablesTrackHash = true;
@@ -184,11 +185,11 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                        sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n,
-                       fromSlots ? sys->pos : nullptr);
+                       fromSlots ? sys->pos : nullptr, sys->hash);
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
-                             sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf);
+                             sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf, sys->hash);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
