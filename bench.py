@@ -69,16 +69,20 @@ def cell_counts(cs: np.ndarray, grid: int) -> tuple[int, int]:
     return int(len(occ)), int(mark.sum())
 
 
-def stage_bytes(name: str, n: int, c_occ: int, c_touch: int) -> int | None:
+def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") -> int | None:
     """Algorithmic HBM bytes of one launch of a stage: every array it must
     touch read or written once, cell tables counted per occupied (or
-    neighbourhood-touched) cell, no dense table clears."""
-    if name == "collide":  # own pos/vel 32 B, index 4 B, newVel 16 B + table probes
-        return 52 * n + 8 * c_touch
-    if name == "collide_integrate":  # + integrate/hash epilogue: pos/vel 32 B, key/value 8 B
-        return 76 * n + 8 * c_touch  # (the newVel store and reload are fused away)
-    if name == "reorder":  # hash+index 8 B, gather 32 B, write 32 B, tables 8 B/cell
-        return 72 * n + 8 * c_occ
+    neighbourhood-touched) cell, no dense table clears.  `path` is "run"
+    (the multi-step run, which keeps the particle state in sorted-slot order
+    between fused steps) or "update" (single updates)."""
+    run = path == "run"
+    if name == "collide":  # own pos/vel 32 B, index 4 B, sorted hash 4 B, newVel 16 B
+        # (+ the positions scattered back to id order by a run's last step)
+        return 56 * n + (16 * n if run else 0) + 8 * c_touch
+    if name == "collide_integrate":  # own pos/vel 32, index 4, hash 4; pos/vel 32, key 4, slot 4
+        return 80 * n + 8 * c_touch  # (the newVel store and reload are fused away)
+    if name == "reorder":  # hash+index 8 B, gather 32 B (+ slot map 4 B), write 32 B, tables
+        return 72 * n + (4 * n if run else 0) + 8 * c_occ
     if name.startswith("sort_pass"):  # keys+values read and written once
         return 16 * n
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
