@@ -40,7 +40,10 @@ constexpr uint64_t kFlagPrefix = 2ull << 62;
 constexpr uint64_t kFlagMask = 3ull << 62;
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr int kLookback = 4;
+#ifndef FC_RS_LOOKBACK
+#define FC_RS_LOOKBACK 4
+#endif
+constexpr int kLookback = FC_RS_LOOKBACK;
 
 __device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t value) {
   return flag | ((uint64_t)(epoch & kEpochMask) << 32) | (uint64_t)value;
