@@ -143,16 +143,19 @@ fc_status ensure_sort_capacity(fc_context* ctx, size_t n) {
   SortScratch& s = ctx->sort;
   size_t tiles = (n + RS_TILE - 1) / RS_TILE;
   if (tiles == 0) tiles = 1;
-  if (s.capacity >= n && s.statusCapacity >= tiles * RS_RADIX) return FC_SUCCESS;
+  if (s.capacity >= n && s.statusCapacity >= tiles * RS_RADIX &&
+      s.groupStatusCapacity >= lookback_groups(tiles) * RS_RADIX)
+    return FC_SUCCESS;
   FC_HIP_TRY(hipStreamSynchronize(ctx->stream));
   hipFree(s.keysA);
   hipFree(s.valsA);
   hipFree(s.keysB);
   hipFree(s.valsB);
   hipFree(s.status);
+  hipFree(s.groupStatus);
   s.keysA = s.valsA = s.keysB = s.valsB = nullptr;
-  s.status = nullptr;
-  s.capacity = s.statusCapacity = 0;
+  s.status = s.groupStatus = nullptr;
+  s.capacity = s.statusCapacity = s.groupStatusCapacity = 0;
   size_t cap = std::max<size_t>(n, RS_TILE);
   FC_HIP_TRY(hipMalloc(&s.keysA, cap * sizeof(uint32_t)));
   FC_HIP_TRY(hipMalloc(&s.valsA, cap * sizeof(uint32_t)));
@@ -161,8 +164,12 @@ fc_status ensure_sort_capacity(fc_context* ctx, size_t n) {
   size_t words = ((cap + RS_TILE - 1) / RS_TILE) * RS_RADIX;
   FC_HIP_TRY(hipMalloc(&s.status, words * sizeof(uint64_t)));
   FC_HIP_TRY(hipMemset(s.status, 0, words * sizeof(uint64_t)));
+  const size_t gwords = lookback_groups((cap + RS_TILE - 1) / RS_TILE) * RS_RADIX;
+  FC_HIP_TRY(hipMalloc(&s.groupStatus, gwords * sizeof(uint64_t)));
+  FC_HIP_TRY(hipMemset(s.groupStatus, 0, gwords * sizeof(uint64_t)));
   s.capacity = cap;
   s.statusCapacity = words;
+  s.groupStatusCapacity = gwords;
   return FC_SUCCESS;
 }
 
@@ -259,6 +266,7 @@ fc_status fc_context_destroy(fc_context* ctx) {
   hipFree(s.keysB);
   hipFree(s.valsB);
   hipFree(s.status);
+  hipFree(s.groupStatus);
   hipFree(s.histogram);
   hipFree(s.counters);
   hipFree(ctx->devFlags);

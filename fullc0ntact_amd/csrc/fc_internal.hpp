@@ -57,7 +57,15 @@ constexpr int RS_RADIX_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_RADIX_BITS;
 constexpr int RS_MAX_PASSES = 4;
 constexpr int RS_MAX_HIST_BLOCKS = 512;
-constexpr int RS_HIST_COPIES = 16;  // privatised copies of the global histogram
+constexpr int RS_HIST_COPIES = 16;
+// Tiles per group of the sort's two-level look-back.
+#ifndef FC_RS_GROUP
+#define FC_RS_GROUP 16
+#endif
+constexpr int RS_LOOKBACK_GROUP = FC_RS_GROUP;
+inline size_t lookback_groups(size_t tiles) {
+  return (tiles + RS_LOOKBACK_GROUP - 1) / RS_LOOKBACK_GROUP;
+}  // privatised copies of the global histogram
 
 // Device counters used by the sort; all self-resetting (see radix_sort.hip).
 enum : int {
@@ -74,6 +82,8 @@ struct SortScratch {
   size_t capacity = 0;               // elements in keysA/valsA/keysB/valsB
   uint64_t* status = nullptr;        // tile x digit look-back words
   size_t statusCapacity = 0;         // in words
+  uint64_t* groupStatus = nullptr;   // tile-group x digit look-back words
+  size_t groupStatusCapacity = 0;    // in words
   uint32_t* histogram = nullptr;     // [RS_MAX_PASSES][RS_HIST_COPIES][RS_RADIX]
   uint32_t* counters = nullptr;      // [RSC_COUNT]
 };

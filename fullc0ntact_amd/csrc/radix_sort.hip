@@ -13,10 +13,20 @@
 //                   from a ticket counter.  Each wave ranks its 1024 keys
 //                   with ballot-based digit matching into a wave-private LDS
 //                   histogram, the block publishes its per-digit counts and
-//                   resolves its global prefix by decoupled look-back over
-//                   the preceding tiles, then stages the tile digit-sorted in
-//                   LDS and writes it out so consecutive lanes write
-//                   consecutive addresses.
+//                   resolves its global prefix with a two-level look-back
+//                   (below), then stages the tile digit-sorted in LDS and
+//                   writes it out so consecutive lanes write consecutive
+//                   addresses.
+//
+// Two-level look-back: tiles form groups of kGroup consecutive tickets.  A
+// tile reads the counts of its group's earlier tiles all at once; the last
+// tile of each group publishes the group's count and, after a decoupled
+// look-back over the earlier groups' words (window kGroupWindow), the
+// group's inclusive prefix.  Every tile finds its group's exclusive prefix
+// by the same group-level look-back.  All tiles of a pass start together,
+// so a tile-level chain would make the last tile walk back through every
+// tile while inclusive prefixes propagate one look-back at a time; the group
+// level shortens that chain kGroup-fold for a few more loads per tile.
 //
 // No memset is needed between sorts: the ticket counters reset themselves
 // (the block holding the last ticket stores 0) and look-back words carry the
@@ -40,10 +50,11 @@ constexpr uint64_t kFlagPrefix = 2ull << 62;
 constexpr uint64_t kFlagMask = 3ull << 62;
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
-#ifndef FC_RS_LOOKBACK
-#define FC_RS_LOOKBACK 4
+constexpr int kGroup = RS_LOOKBACK_GROUP;  // tiles per look-back group
+#ifndef FC_RS_GROUP_WINDOW
+#define FC_RS_GROUP_WINDOW 4
 #endif
-constexpr int kLookback = FC_RS_LOOKBACK;
+constexpr int kGroupWindow = FC_RS_GROUP_WINDOW;  // group words per round trip
 
 __device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t value) {
   return flag | ((uint64_t)(epoch & kEpochMask) << 32) | (uint64_t)value;
@@ -95,8 +106,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_histogram(
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int pass,
-    int passes, uint32_t* gHist, uint64_t* status, uint32_t* counters, uint32_t* devFlags,
-    const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
+    int passes, uint32_t* gHist, uint64_t* status, uint64_t* groupStatus, uint32_t* counters,
+    uint32_t* devFlags, const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
     uint32_t* __restrict__ cellEnd) {
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
@@ -123,11 +134,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     uint32_t* h = hist_buffer(gHist, 0);
     for (int i = threadIdx.x; i < RS_HIST_COPIES * RS_RADIX; i += RS_THREADS) h[i] = 0u;
   }
-  // Global bucket start of this thread's digit (threads 0..255).
-  const uint32_t digitTotal =
-      threadIdx.x < (uint32_t)RS_RADIX ? histogram_total(gHist, pass, threadIdx.x) : 0u;
   __syncthreads();
-  const uint32_t digitGlobal = scan256_exclusive(digitTotal, s_scratch);
   const uint32_t tile = s_tile;
   const uint32_t epoch = (counters[RSC_EPOCH] * 4u + (uint32_t)pass) & kEpochMask;
   const uint32_t tileBase = tile * RS_TILE;
@@ -157,6 +164,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     key[i] = valid ? keysIn[idx] : 0xffffffffu;
     val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
   }
+  // Global total of this thread's digit (threads 0..255), requested while
+  // the keys are in flight.
+  const uint32_t digitTotal =
+      threadIdx.x < (uint32_t)RS_RADIX ? histogram_total(gHist, pass, threadIdx.x) : 0u;
   // Rank within the wave, slice by slice in index order (stable), and count
   // the next pass's digit.
   const bool countNext = pass + 1 < passes;
@@ -189,36 +200,69 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       s_waveHist[w][d] = tileCount;
       tileCount += c;
     }
-    __hip_atomic_store(myStatus, pack_status(tile == 0u ? kFlagPrefix : kFlagAgg, epoch, tileCount),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(myStatus, pack_status(kFlagAgg, epoch, tileCount), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
   const uint32_t tileStart = scan256_exclusive(digitThread ? tileCount : 0u, s_scratch);
+  const uint32_t digitGlobal = scan256_exclusive(digitTotal, s_scratch);
   if (digitThread) {
     s_tileStart[d] = tileStart;
-    // Decoupled look-back for this digit, reading kLookback predecessor
-    // words per round trip and consuming them newest to oldest until an
-    // inclusive prefix is found.
-    uint32_t prefix = 0;
-    if (tile > 0u) {
-      int t = (int)tile - 1;
-      uint32_t spins = 0;
-      for (;;) {
-        uint64_t w[kLookback];
+    bool stalled = false;
+    // Counts of the group's earlier tiles, all requested at once.
+    const uint32_t g = tile / (uint32_t)kGroup, first = g * (uint32_t)kGroup;
+    const uint32_t nIn = tile - first;
+    uint64_t w[kGroup - 1];
 #pragma unroll
-        for (int i = 0; i < kLookback; ++i)
-          w[i] = (t - i >= 0) ? __hip_atomic_load(status + (size_t)(t - i) * RS_RADIX + d,
+    for (int i = 0; i < kGroup - 1; ++i)
+      w[i] = (uint32_t)i < nIn ? __hip_atomic_load(status + (size_t)(first + i) * RS_RADIX + d,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0ull;
+    uint32_t inGroup = 0;
+#pragma unroll
+    for (int i = 0; i < kGroup - 1; ++i) {
+      if ((uint32_t)i < nIn) {
+        uint32_t spins = 0;
+        while (!stalled && ((w[i] & kFlagMask) == 0ull ||
+                            ((uint32_t)(w[i] >> 32) & kEpochMask) != epoch)) {
+          if (++spins > kSpinLimit) {
+            stalled = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          w[i] = __hip_atomic_load(status + (size_t)(first + i) * RS_RADIX + d, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        inGroup += (uint32_t)w[i];
+      }
+    }
+    uint64_t* gWord = groupStatus + (size_t)g * RS_RADIX + d;
+    const bool leader = nIn == (uint32_t)kGroup - 1u;
+    if (leader)
+      __hip_atomic_store(gWord, pack_status(g == 0u ? kFlagPrefix : kFlagAgg, epoch, inGroup + tileCount),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Decoupled look-back over the earlier groups, kGroupWindow words per
+    // round trip, consumed newest to oldest until an inclusive prefix.
+    uint32_t before = 0;
+    if (g > 0u) {
+      int t = (int)g - 1;
+      uint32_t spins = 0;
+      while (!stalled) {
+        uint64_t v[kGroupWindow];
+#pragma unroll
+        for (int i = 0; i < kGroupWindow; ++i)
+          v[i] = (t - i >= 0) ? __hip_atomic_load(groupStatus + (size_t)(t - i) * RS_RADIX + d,
                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : 0ull;
         int used = 0;
         bool done = false;
 #pragma unroll
-        for (int i = 0; i < kLookback; ++i) {
+        for (int i = 0; i < kGroupWindow; ++i) {
           if (done || used != i) break;
-          if (t - i < 0) { done = true; break; }  // unreachable: tile 0 holds a prefix
-          const uint64_t flag = w[i] & kFlagMask;
-          const uint32_t wEpoch = (uint32_t)(w[i] >> 32) & kEpochMask;
-          if (flag == 0ull || wEpoch != epoch) break;  // not published yet
-          prefix += (uint32_t)w[i];
+          if (t - i < 0) { done = true; break; }  // unreachable: group 0 holds a prefix
+          const uint64_t flag = v[i] & kFlagMask;
+          const uint32_t vEpoch = (uint32_t)(v[i] >> 32) & kEpochMask;
+          if (flag == 0ull || vEpoch != epoch) break;  // not published yet
+          before += (uint32_t)v[i];
           ++used;
           if (flag == kFlagPrefix) done = true;
         }
@@ -226,15 +270,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         t -= used;
         if (used == 0) {
           if (++spins > kSpinLimit) {
-            atomicOr(devFlags, DEVERR_SORT_STALLED);
+            stalled = true;
             break;
           }
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      __hip_atomic_store(myStatus, pack_status(kFlagPrefix, epoch, prefix + tileCount),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (leader)
+        __hip_atomic_store(gWord, pack_status(kFlagPrefix, epoch, before + inGroup + tileCount),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (stalled) atomicOr(devFlags, DEVERR_SORT_STALLED);
+    const uint32_t prefix = before + inGroup;
     s_digitBase[d] = digitGlobal + prefix - tileStart;
   }
   __syncthreads();
@@ -302,7 +349,9 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
   if (clearHash != nullptr && passes < 2) return hipErrorInvalidValue;
   if (passes > RS_MAX_PASSES) return hipErrorInvalidValue;
   const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
-  if (s.capacity < n || s.statusCapacity < (size_t)numTiles * RS_RADIX) return hipErrorInvalidValue;
+  if (s.capacity < n || s.statusCapacity < (size_t)numTiles * RS_RADIX ||
+      s.groupStatusCapacity < (size_t)lookback_groups(numTiles) * RS_RADIX)
+    return hipErrorInvalidValue;
   hipError_t e = hipSuccess;
   if (!histogramReady) {
     uint32_t histBlocks = numTiles;
@@ -336,7 +385,8 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
                                                "sort_pass3"};
     if (timer) timer->begin(stream, names[p]);
     hipLaunchKernelGGL(k_rs_scatter, dim3(numTiles), dim3(RS_THREADS), 0, stream, kin, vin, kout,
-                       vout, n, p, passes, s.histogram, s.status, s.counters, devFlags,
+                       vout, n, p, passes, s.histogram, s.status, s.groupStatus, s.counters,
+                       devFlags,
                        p == 0 ? clearHash : nullptr, cellStart, cellEnd);
     e = hipGetLastError();
     if (timer) timer->end(stream);
