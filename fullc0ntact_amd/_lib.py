@@ -156,6 +156,14 @@ _SIGS = {
     "fc_distmap_download": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "fc_distmap_query": (_st, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "fc_distmap_eval": (_st, [_vp, _vp, _vp, _u32, C.POINTER(TransInfo), _vp, _vp, _vp]),
+    "fc_distmap_file_read": (_st, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int32), _vp, _vp, _vp, _vp, _vp]),
+    "fc_distmap_load": (_st, [_vp, C.c_char_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                              C.POINTER(_vp)]),
+    "fc_distmap_save": (_st, [_vp, _vp, C.c_char_p]),
+    "fc_distmap_get_fbm": (_st, [_vp, _vp]),
+    "fc_distmap_set_fbm": (_st, [_vp, _vp]),
+    "fc_distmap_query_fbm": (_st, [_vp, _vp, _vp, _u32, _vp]),
     "fc_integrate_system_distmap": (_st, [_vp, _vp, _vp, _vp, C.c_float, _u32]),
     "fc_mesh_signed_distance": (_st, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp]),
     "fc_system_set_polydisperse": (_st, [_vp, C.c_int]),

@@ -4,7 +4,13 @@
 #include "distmap.hpp"
 
 #include <cfloat>
+#include <charconv>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iterator>
+#include <new>
+#include <string>
 #include <vector>
 
 namespace fc {
@@ -19,6 +25,21 @@ inline unsigned grid_for(size_t n, int block) {
 }
 
 // Batched queryMap over float[3] points (the reference's vector3 arrays).
+// queryFBM (distancemap.cuh:195-225): 1 where the interpolated distance of
+// a point inside the box is <= 0, else 0 (0 outside the box).
+__global__ __launch_bounds__(kBlock) void k_distmap_query_fbm(DistMapView m,
+                                                              const float* __restrict__ pts,
+                                                              uint32_t n,
+                                                              int32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const v3f q = make_v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  float d = 0.0f;
+  v3f c = make_v3(0.0f, 0.0f, 0.0f);
+  const bool inside = distmap_query(m, q, d, c, nullptr);
+  out[i] = (inside && d <= 0.0f) ? 1 : 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_distmap_query(DistMapView m,
                                                           const float* __restrict__ pts,
                                                           uint32_t n, float* __restrict__ dist,
@@ -330,6 +351,7 @@ struct fc_distmap {
   float4* cpd = nullptr;
   float4* nrm = nullptr;
   size_t V = 0;
+  std::vector<int32_t> fbm;  // stateFBM_ per vertex (host; empty = all 0)
 };
 
 namespace fc {
@@ -378,9 +400,231 @@ fc_status alloc_map(fc_context* ctx, const int32_t cells[3], fc_distmap** out) {
   return FC_SUCCESS;
 }
 
+// Text map file of RigidBody::buildDistanceMapFromFile /
+// storeDistanceMapToFile (rigidbody.cpp:1214-1376): "dim0 dim1", "cx cy cz",
+// "cellSize", then (cx+1)(cy+1)(cz+1) lines each of vertex coordinates,
+// closest points, normals (3 values), distances and FBM states (1 value).
+// Like the reference's stream reads, each record's numbers are read across
+// whitespace and line breaks and the rest of its last line is skipped.
+struct MapFileReader {
+  std::string text;
+  size_t at = 0;
+  bool ok = true;
+
+  void skip_space() {
+    while (at < text.size() && (text[at] == ' ' || text[at] == '\t' || text[at] == '\n' ||
+                                text[at] == '\r' || text[at] == '\v' || text[at] == '\f'))
+      ++at;
+  }
+  void end_line() {
+    while (at < text.size() && text[at] != '\n') ++at;
+    if (at < text.size()) ++at;
+  }
+  double real() {
+    skip_space();
+    if (at >= text.size()) { ok = false; return 0.0; }
+    double v = 0.0;
+    const char* b = text.data() + at;
+    const char* e = text.data() + text.size();
+    if (*b == '+') ++b;  // from_chars takes no leading plus; stream extraction does
+    auto r = std::from_chars(b, e, v);
+    if (r.ec != std::errc() && r.ec != std::errc::result_out_of_range) { ok = false; return 0.0; }
+    at = (size_t)(r.ptr - text.data());
+    return v;
+  }
+  long long integer() {
+    skip_space();
+    if (at >= text.size()) { ok = false; return 0; }
+    long long v = 0;
+    const char* b = text.data() + at;
+    const char* e = text.data() + text.size();
+    if (*b == '+') ++b;
+    auto r = std::from_chars(b, e, v);
+    if (r.ec != std::errc()) { ok = false; return 0; }
+    at = (size_t)(r.ptr - text.data());
+    return v;
+  }
+};
+
+struct MapFileData {
+  int32_t dim[2] = {0, 0};
+  int32_t cells[3] = {0, 0, 0};
+  double cellSize = 0.0;
+  std::vector<double> coords, cps, normals, dist;  // 3, 3, 3, 1 per vertex
+  std::vector<int32_t> fbm;
+};
+
+fc_status read_map_file(const char* path, bool headerOnly, MapFileData& d) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f.is_open()) return set_error(FC_ERROR_INVALID_VALUE, std::string("cannot open ") + path);
+  MapFileReader r;
+  try {
+    r.text.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  } catch (const std::bad_alloc&) {
+    return set_error(FC_ERROR_OUT_OF_MEMORY, "map file: text buffer");
+  }
+  d.dim[0] = (int32_t)r.integer();
+  d.dim[1] = (int32_t)r.integer();
+  r.end_line();
+  for (int a = 0; a < 3; ++a) d.cells[a] = (int32_t)r.integer();
+  r.end_line();
+  d.cellSize = r.real();
+  r.end_line();
+  if (!r.ok) return set_error(FC_ERROR_INVALID_VALUE, "map file: bad header");
+  for (int a = 0; a < 3; ++a)
+    if (d.cells[a] < 1 || d.cells[a] > 4096)
+      return set_error(FC_ERROR_INVALID_VALUE, "map file: cell counts outside [1, 4096]");
+  if (headerOnly) return FC_SUCCESS;
+  const size_t V = (size_t)(d.cells[0] + 1) * (d.cells[1] + 1) * (d.cells[2] + 1);
+  if (V > (size_t(1) << 27)) return set_error(FC_ERROR_CAPACITY, "map file: more than 2^27 vertices");
+  try {
+    d.coords.resize(3 * V);
+    d.cps.resize(3 * V);
+    d.normals.resize(3 * V);
+    d.dist.resize(V);
+    d.fbm.resize(V);
+  } catch (const std::bad_alloc&) {
+    return set_error(FC_ERROR_OUT_OF_MEMORY, "map file: host arrays");
+  }
+  std::vector<double>* tri[3] = {&d.coords, &d.cps, &d.normals};
+  for (auto* t : tri) {
+    for (size_t v = 0; v < V && r.ok; ++v) {
+      (*t)[3 * v] = r.real();
+      (*t)[3 * v + 1] = r.real();
+      (*t)[3 * v + 2] = r.real();
+      r.end_line();
+    }
+  }
+  for (size_t v = 0; v < V && r.ok; ++v) {
+    d.dist[v] = r.real();
+    r.end_line();
+  }
+  for (size_t v = 0; v < V && r.ok; ++v) {
+    d.fbm[v] = (int32_t)r.integer();
+    r.end_line();
+  }
+  if (!r.ok) return set_error(FC_ERROR_INVALID_VALUE, "map file: truncated or malformed data");
+  return FC_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
+
+fc_status fc_distmap_file_read(const char* path, int32_t cells[3], double* cellSize,
+                               int32_t dim[2], double* vertexCoords, double* contactPoints,
+                               double* normals, double* distance, int32_t* stateFBM) {
+  if (!path || !cells || !cellSize) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  const bool headerOnly =
+      !vertexCoords && !contactPoints && !normals && !distance && !stateFBM;
+  MapFileData d;
+  FC_TRY(read_map_file(path, headerOnly, d));
+  for (int a = 0; a < 3; ++a) cells[a] = d.cells[a];
+  *cellSize = d.cellSize;
+  if (dim) {
+    dim[0] = d.dim[0];
+    dim[1] = d.dim[1];
+  }
+  if (headerOnly) return FC_SUCCESS;
+  const size_t V = d.dist.size();
+  if (vertexCoords) std::copy(d.coords.begin(), d.coords.end(), vertexCoords);
+  if (contactPoints) std::copy(d.cps.begin(), d.cps.end(), contactPoints);
+  if (normals) std::copy(d.normals.begin(), d.normals.end(), normals);
+  if (distance) std::copy(d.dist.begin(), d.dist.end(), distance);
+  if (stateFBM) std::copy(d.fbm.begin(), d.fbm.begin() + V, stateFBM);
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_load(fc_context* ctx, const char* path, const float bmin[3],
+                          const float bmax[3], fc_distmap** out) {
+  if (!ctx || !path || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if ((bmin == nullptr) != (bmax == nullptr))
+    return set_error(FC_ERROR_INVALID_VALUE, "give both box corners or neither");
+  *out = nullptr;
+  MapFileData d;
+  FC_TRY(read_map_file(path, false, d));
+  const size_t V = d.dist.size();
+  // binary64 -> binary32 as the conversion constructor does
+  // (DistanceMap<float,cpu>(DistanceMap<Real,cpu>*), distancemap.cpp:129-178)
+  std::vector<float> c, p, nn, ds;
+  try {
+    c.resize(3 * V);
+    p.resize(3 * V);
+    nn.resize(3 * V);
+    ds.resize(V);
+  } catch (const std::bad_alloc&) {
+    return set_error(FC_ERROR_OUT_OF_MEMORY, "map file: float arrays");
+  }
+  for (size_t i = 0; i < 3 * V; ++i) {
+    c[i] = (float)d.coords[i];
+    p[i] = (float)d.cps[i];
+    nn[i] = (float)d.normals[i];
+  }
+  for (size_t v = 0; v < V; ++v) ds[v] = (float)d.dist[v];
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = bmin ? bmin[a] : c[a];
+    hi[a] = bmax ? bmax[a] : c[3 * (V - 1) + a];
+  }
+  fc_distmap* m = nullptr;
+  FC_TRY(fc_distmap_create(ctx, d.cells, (float)d.cellSize, lo, hi, c.data(), nn.data(), p.data(),
+                           ds.data(), &m));
+  m->fbm = std::move(d.fbm);
+  *out = m;
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_save(fc_context* ctx, const fc_distmap* m, const char* path) {
+  if (!ctx || !m || !path) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  const size_t V = m->V;
+  std::vector<float> c(3 * V), p(3 * V), nn(3 * V), ds(V);
+  FC_TRY(fc_distmap_download(ctx, m, c.data(), nn.data(), p.data(), ds.data()));
+  std::FILE* f = std::fopen(path, "w");
+  if (!f) return set_error(FC_ERROR_INVALID_VALUE, std::string("cannot write ") + path);
+  // %.9g: nine significant digits give every binary32 value back exactly
+  // (the reference writes six).
+  std::fprintf(f, "%d %d\n", m->cells[0] + 1, (m->cells[1] + 1) * (m->cells[2] + 1));
+  std::fprintf(f, "%d %d %d\n", m->cells[0], m->cells[1], m->cells[2]);
+  std::fprintf(f, "%.9g\n", (double)m->cellSize);
+  const float* tri[3] = {c.data(), p.data(), nn.data()};
+  for (const float* t : tri)
+    for (size_t v = 0; v < V; ++v)
+      std::fprintf(f, "%.9g %.9g %.9g\n", (double)t[3 * v], (double)t[3 * v + 1],
+                   (double)t[3 * v + 2]);
+  for (size_t v = 0; v < V; ++v) std::fprintf(f, "%.9g\n", (double)ds[v]);
+  for (size_t v = 0; v < V; ++v) std::fprintf(f, "%d\n", m->fbm.empty() ? 0 : (int)m->fbm[v]);
+  const bool bad = std::ferror(f) != 0;
+  if (std::fclose(f) != 0 || bad) return set_error(FC_ERROR_INVALID_VALUE, "map file write failed");
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_get_fbm(const fc_distmap* m, int32_t* stateFBM) {
+  if (!m || !stateFBM) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  for (size_t v = 0; v < m->V; ++v) stateFBM[v] = m->fbm.empty() ? 0 : m->fbm[v];
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_set_fbm(fc_distmap* m, const int32_t* stateFBM) {
+  if (!m) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (!stateFBM) {
+    m->fbm.clear();
+    return FC_SUCCESS;
+  }
+  m->fbm.assign(stateFBM, stateFBM + m->V);
+  return FC_SUCCESS;
+}
+
+fc_status fc_distmap_query_fbm(fc_context* ctx, const fc_distmap* m, const float* points,
+                               uint32_t n, int32_t* out) {
+  if (!ctx || !m) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (n == 0) return FC_SUCCESS;
+  if (!points || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  hipLaunchKernelGGL(k_distmap_query_fbm, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                     ctx->stream, distmap_view(m), points, n, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(e, "distmap queryFBM");
+  return FC_SUCCESS;
+}
 
 fc_status fc_distmap_create(fc_context* ctx, const int32_t cells[3], float cellSize,
                             const float bmin[3], const float bmax[3], const float* vertexCoords,

@@ -269,6 +269,37 @@ fc_status fc_distmap_build(fc_context* ctx, const double* vertices, uint32_t num
                            const int32_t cells[3], float cellSize, const float bmin[3],
                            fc_distmap** out);
 fc_status fc_distmap_destroy(fc_distmap* map);
+/* Map files of RigidBody::buildDistanceMapFromFile / storeDistanceMapToFile
+ * (rigidbody.cpp:1214-1376): text lines "dim0 dim1", "cx cy cz",
+ * "cellSize", then per lattice vertex (x fastest) the coordinates, closest
+ * points and normals (3 values each), the distances and the FBM states.
+ *
+ * fc_distmap_file_read parses one into host binary64 arrays (the reference
+ * reads into DistanceMap<Real>): with all array pointers NULL it returns
+ * only the header, so callers can size the arrays ((cx+1)(cy+1)(cz+1)
+ * vertices). */
+fc_status fc_distmap_file_read(const char* path, int32_t cells[3], double* cellSize,
+                               int32_t dim[2], double* vertexCoords, double* contactPoints,
+                               double* normals, double* distance, int32_t* stateFBM);
+/* Loads a map file onto the device, converting binary64 to binary32 as
+ * DistanceMap<float,cpu>(DistanceMap<Real,cpu>*) does (distancemap.cpp:
+ * 129-178).  The file holds no bounding box (the reference takes it from
+ * the body's AABB): pass bmin/bmax, or NULL for both to use the first and
+ * the last lattice vertex.  The FBM states are kept with the map. */
+fc_status fc_distmap_load(fc_context* ctx, const char* path, const float bmin[3],
+                          const float bmax[3], fc_distmap** out);
+/* Writes a map in the same format, with nine significant digits so that
+ * every binary32 value reads back exactly (the reference writes six). */
+fc_status fc_distmap_save(fc_context* ctx, const fc_distmap* map, const char* path);
+/* FBM states of the lattice vertices (host, one int per vertex; 0 when
+ * none were loaded or set; NULL in set clears them). */
+fc_status fc_distmap_get_fbm(const fc_distmap* map, int32_t* stateFBM);
+fc_status fc_distmap_set_fbm(fc_distmap* map, const int32_t* stateFBM);
+/* queryFBM (distancemap.cuh:195-225) for n points (device arrays, 3 floats
+ * per point): 1 where the interpolated distance of a point inside the box
+ * is <= 0, else 0. */
+fc_status fc_distmap_query_fbm(fc_context* ctx, const fc_distmap* map, const float* points,
+                               uint32_t n, int32_t* out);
 fc_status fc_distmap_info(const fc_distmap* map, int32_t cells[3], float* cellSize, float bmin[3],
                           float bmax[3]);
 /* Copies a map back in the reference's host layout (any pointer may be
