@@ -164,6 +164,7 @@ _SIGS = {
     "fc_distmap_get_fbm": (_st, [_vp, _vp]),
     "fc_distmap_set_fbm": (_st, [_vp, _vp]),
     "fc_distmap_query_fbm": (_st, [_vp, _vp, _vp, _u32, _vp]),
+    "fc_write_particle_vtk": (_st, [C.c_char_p, _vp, _u32, C.c_float, C.c_int]),
     "fc_integrate_system_distmap": (_st, [_vp, _vp, _vp, _vp, C.c_float, _u32]),
     "fc_mesh_signed_distance": (_st, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp]),
     "fc_system_set_polydisperse": (_st, [_vp, C.c_int]),

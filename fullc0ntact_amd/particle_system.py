@@ -158,6 +158,14 @@ class ParticleSystem:
         self.getArray(self.POSITION)
         self.getArray(self.VELOCITY)
 
+    def writeParticleVtk(self, path, radius_from_w: bool = False) -> None:
+        """The particle dump of particles_gpu (WriteGPUParticleFile,
+        vtkwriter.cpp:3399-3446) of the current positions, with the
+        system's radius (or pos.w) in the radius column."""
+        from .scene_io import write_particle_vtk
+        write_particle_vtk(path, self.getArray(self.POSITION), self.getParticleRadius(),
+                           radius_from_w)
+
     def dumpParticles(self, start: int, count: int) -> None:
         self.transferArrays()
         for i in range(start, start + count):

@@ -490,6 +490,18 @@ fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrog
                                uint8_t* candidate, uint8_t* contact, uint32_t counts[3]);
 
 /* ------------------------------------------------------------------------- */
+/* Output files.                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* CVtkWriter::WriteGPUParticleFile (vtkwriter.cpp:3399-3446): legacy ASCII
+ * VTK unstructured grid of n sphere centres (host float4 array) with the
+ * point scalars "radius" and "myid", every value printed with %f.  The
+ * reference writes the radius 1/64 for every sphere: pass radius = 1/64 and
+ * radiusFromW = 0 for identical files, or radiusFromW != 0 to write pos.w. */
+fc_status fc_write_particle_vtk(const char* path, const float* pos4, uint32_t n, float radius,
+                                int radiusFromW);
+
+/* ------------------------------------------------------------------------- */
 /* Per-stage timing with HIP events on the context stream.                   */
 /* ------------------------------------------------------------------------- */
 

@@ -198,6 +198,14 @@ class ParticleSystem {
                   m_hVel[4 * i + 2], m_hVel[4 * i + 3]);
     }
   }
+  // The particle dump of particles_gpu (CVtkWriter::WriteGPUParticleFile,
+  // vtkwriter.cpp:3399-3446) of the current positions; the radius column is
+  // the system's radius, or pos.w for polydisperse systems.
+  void writeParticleVtk(const char* path, bool radiusFromW = false) {
+    getArray(POSITION);
+    check(fc_write_particle_vtk(path, m_hPos.data(), m_numParticles, m_params.particleRadius,
+                                radiusFromW ? 1 : 0));
+  }
   // transferArrays (particleSystem.cpp:316-321): refreshes the host mirrors.
   void transferArrays(uint32_t /*start*/, uint32_t /*count*/) {
     getArray(POSITION);
