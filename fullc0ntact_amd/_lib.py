@@ -165,6 +165,16 @@ _SIGS = {
     "fc_distmap_set_fbm": (_st, [_vp, _vp]),
     "fc_distmap_query_fbm": (_st, [_vp, _vp, _vp, _u32, _vp]),
     "fc_write_particle_vtk": (_st, [C.c_char_p, _vp, _u32, C.c_float, C.c_int]),
+    "fc_mesh_mesh_contacts": (_st, [_vp, _vp, _vp, _u32, C.POINTER(TransInfo), C.c_float, _vp,
+                                    _vp, _vp, _u32, C.POINTER(_u32)]),
+    "fc_coupling_create": (_st, [_vp, _u32, _vp, UInt3, C.POINTER(_vp)]),
+    "fc_coupling_destroy": (_st, [_vp]),
+    "fc_coupling_system": (_vp, [_vp]),
+    "fc_coupling_set_time_step": (_st, [_vp, C.c_double]),
+    "fc_coupling_step": (_st, [_vp, C.c_double, _u32]),
+    "fc_coupling_get_body": (_st, [_vp, _u32, _vp, _vp]),
+    "fc_coupling_set_body": (_st, [_vp, _u32, _vp, _vp]),
+    "fc_featflow_bind": (None, [_vp]),
     "fc_integrate_system_distmap": (_st, [_vp, _vp, _vp, _vp, C.c_float, _u32]),
     "fc_mesh_signed_distance": (_st, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp]),
     "fc_system_set_polydisperse": (_st, [_vp, C.c_int]),

@@ -80,23 +80,21 @@ __device__ __forceinline__ v3f mat3_mul(const float* M, v3f v) {
 }
 
 // eval_distmap_kernel (difi.cu:427-474).
-__global__ __launch_bounds__(kBlock) void k_distmap_eval(DistMapView m, fc_trans_info T,
-                                                         const float* __restrict__ v, uint32_t n,
-                                                         float* __restrict__ cps,
-                                                         float* __restrict__ normals,
-                                                         float* __restrict__ distance) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const v3f p = make_v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+// eval_distmap for one vertex p of body 1 (difi.cu:427-474): to world space
+// by body 1's pose, into body 0's model frame, queryMap there, then the
+// world closest point, the unit normal from the vertex to it and the
+// midpoint contact.  Returns false outside body 0's box (distance 1000).
+__device__ __forceinline__ bool eval_vertex(const DistMapView& m, const fc_trans_info& T, v3f p,
+                                            float& d, v3f& contact, v3f& normal) {
   v3f qw = mat3_mul(T.m2w1, p);
   qw = make_v3(qw.x + T.origin1.x, qw.y + T.origin1.y, qw.z + T.origin1.z);
   const v3f q = mat3_mul(T.w2m0, make_v3(qw.x - T.origin0.x, qw.y - T.origin0.y, qw.z - T.origin0.z));
   if (!(m.bmin[0] <= q.x && q.x <= m.bmax[0] && m.bmin[1] <= q.y && q.y <= m.bmax[1] &&
         m.bmin[2] <= q.z && q.z <= m.bmax[2])) {
-    distance[i] = 1000.0f;
-    return;
+    d = 1000.0f;
+    return false;
   }
-  float d = 1000.0f;
+  d = 1000.0f;
   v3f cp = make_v3(0.0f, 0.0f, 0.0f);
   distmap_query(m, q, d, cp, nullptr);
   v3f c0 = mat3_mul(T.m2w0, cp);
@@ -105,13 +103,86 @@ __global__ __launch_bounds__(kBlock) void k_distmap_eval(DistMapView m, fc_trans
   const v3f nn = make_v3(c0.x - qw.x, c0.y - qw.y, c0.z - qw.z);
   const float mag = sqrtf(dot3(nn, nn));
   const float invMag = (float)(1.0 / (double)mag);
-  normals[3 * i] = nn.x * invMag;
-  normals[3 * i + 1] = nn.y * invMag;
-  normals[3 * i + 2] = nn.z * invMag;
-  cps[3 * i] = (c0.x + qw.x) * 0.5f;
-  cps[3 * i + 1] = (c0.y + qw.y) * 0.5f;
-  cps[3 * i + 2] = (c0.z + qw.z) * 0.5f;
+  normal = make_v3(nn.x * invMag, nn.y * invMag, nn.z * invMag);
+  contact = make_v3((c0.x + qw.x) * 0.5f, (c0.y + qw.y) * 0.5f, (c0.z + qw.z) * 0.5f);
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_distmap_eval(DistMapView m, fc_trans_info T,
+                                                         const float* __restrict__ v, uint32_t n,
+                                                         float* __restrict__ cps,
+                                                         float* __restrict__ normals,
+                                                         float* __restrict__ distance) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float d;
+  v3f c, nn;
+  if (eval_vertex(m, T, make_v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]), d, c, nn)) {
+    normals[3 * i] = nn.x;
+    normals[3 * i + 1] = nn.y;
+    normals[3 * i + 2] = nn.z;
+    cps[3 * i] = c.x;
+    cps[3 * i + 1] = c.y;
+    cps[3 * i + 2] = c.z;
+  }
   distance[i] = d;
+}
+
+// ColliderMeshMesh<gpu>::collide (collidermeshmesh.cuh:89-172) keeps the
+// vertices with eval distance < threshold as contacts, in vertex order.
+// Pass 1 counts them per block; pass 2 writes each at its block's offset
+// plus its rank in the block (ballots), so the list is in vertex order.
+__global__ __launch_bounds__(kBlock) void k_mesh_contacts_count(DistMapView m, fc_trans_info T,
+                                                                const float* __restrict__ v,
+                                                                uint32_t n, float threshold,
+                                                                uint32_t* __restrict__ blockCounts) {
+  __shared__ uint32_t s_w[kBlock / 64];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool hit = false;
+  if (i < n) {
+    float d;
+    v3f c, nn;
+    eval_vertex(m, T, make_v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]), d, c, nn);
+    hit = d < threshold;
+  }
+  const uint64_t b = __ballot(hit);
+  if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_w[w];
+    blockCounts[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mesh_contacts_write(
+    DistMapView m, fc_trans_info T, const float* __restrict__ v, uint32_t n, float threshold,
+    const uint32_t* __restrict__ blockOffsets, uint32_t capacity, float* __restrict__ cps,
+    float* __restrict__ normals, uint32_t* __restrict__ ids) {
+  __shared__ uint32_t s_w[kBlock / 64];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  bool hit = false;
+  float d = 1000.0f;
+  v3f c = make_v3(0.0f, 0.0f, 0.0f), nn = c;
+  if (i < n) {
+    eval_vertex(m, T, make_v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]), d, c, nn);
+    hit = d < threshold;
+  }
+  const uint64_t b = __ballot(hit);
+  if (lane == 0u) s_w[wave] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t rank = blockOffsets[blockIdx.x] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+  for (uint32_t w = 0; w < wave; ++w) rank += s_w[w];
+  if (hit && rank < capacity) {
+    cps[3 * (size_t)rank] = c.x;
+    cps[3 * (size_t)rank + 1] = c.y;
+    cps[3 * (size_t)rank + 2] = c.z;
+    normals[3 * (size_t)rank] = nn.x;
+    normals[3 * (size_t)rank + 1] = nn.y;
+    normals[3 * (size_t)rank + 2] = nn.z;
+    ids[rank] = i;
+  }
 }
 
 // integrateSystem with the distance-map boundary on caller arrays.
@@ -814,6 +885,49 @@ fc_status fc_distmap_eval(fc_context* ctx, const fc_distmap* m, const float* v, 
                      distmap_view(m), *info, v, n, cps, normals, distance);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_error(e, "eval_distmap");
+  return FC_SUCCESS;
+}
+
+fc_status fc_mesh_mesh_contacts(fc_context* ctx, const fc_distmap* m, const float* v, uint32_t n,
+                                const fc_trans_info* info, float threshold, float* cps,
+                                float* normals, uint32_t* vertexIds, uint32_t capacity,
+                                uint32_t* numContacts) {
+  if (!ctx || !m || !info || !numContacts) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  *numContacts = 0;
+  if (n == 0) return FC_SUCCESS;
+  if (!v || (capacity > 0 && (!cps || !normals || !vertexIds)))
+    return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  const uint32_t blocks = grid_for(n, kBlock);
+  uint32_t* dCounts = nullptr;
+  hipError_t e = hipMalloc(&dCounts, (size_t)blocks * sizeof(uint32_t));
+  if (e != hipSuccess) return hip_error(e, "contact counts");
+  hipStream_t st = ctx->stream;
+  const DistMapView view = distmap_view(m);
+  hipLaunchKernelGGL(k_mesh_contacts_count, dim3(blocks), dim3(kBlock), 0, st, view, *info, v, n,
+                     threshold, dCounts);
+  e = hipGetLastError();
+  std::vector<uint32_t> counts(blocks);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(counts.data(), dCounts, (size_t)blocks * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  uint64_t total = 0;
+  for (uint32_t b = 0; b < blocks; ++b) {
+    const uint32_t c = counts[b];
+    counts[b] = (uint32_t)total;
+    total += c;
+  }
+  if (e == hipSuccess && total > 0 && capacity > 0) {
+    e = hipMemcpyAsync(dCounts, counts.data(), (size_t)blocks * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_mesh_contacts_write, dim3(blocks), dim3(kBlock), 0, st, view, *info, v,
+                         n, threshold, dCounts, capacity, cps, normals, vertexIds);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  hipFree(dCounts);
+  if (e != hipSuccess) return hip_error(e, "mesh-mesh contacts");
+  *numContacts = (uint32_t)total;
   return FC_SUCCESS;
 }
 

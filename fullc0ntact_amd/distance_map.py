@@ -143,6 +143,27 @@ class DistanceMap:
         self.ctx.sync()
         return do.download(np.int32, n)
 
+    def mesh_contacts(self, vertices, info: _lib.TransInfo, threshold: float = 0.0015,
+                      capacity: int | None = None):
+        """ColliderMeshMesh<gpu>::collide: the vertices of body 1 (model
+        frame, host (n,3)) closer than `threshold` to this map's body, in
+        vertex order: (contact points, normals, vertex ids)."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        n = len(v)
+        cap = n if capacity is None else int(capacity)
+        dv = DeviceArray.from_host(self.ctx, v)
+        dc = DeviceArray(self.ctx, 12 * max(cap, 1))
+        dn = DeviceArray(self.ctx, 12 * max(cap, 1))
+        di = DeviceArray(self.ctx, 4 * max(cap, 1))
+        cnt = C.c_uint32()
+        check(lib.fc_mesh_mesh_contacts(self.ctx.handle, self.handle, dv.ptr, n, C.byref(info),
+                                        float(threshold), dc.ptr, dn.ptr, di.ptr, cap,
+                                        C.byref(cnt)))
+        k = min(cnt.value, cap)
+        return (dc.download(np.float32, 3 * k).reshape(k, 3),
+                dn.download(np.float32, 3 * k).reshape(k, 3), di.download(np.uint32, k),
+                cnt.value)
+
     def eval(self, points, info: _lib.TransInfo):
         """eval_distmap: returns (cps, normals, distance); cps/normals are 0
         where the point is outside the box (distance 1000)."""

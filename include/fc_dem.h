@@ -316,6 +316,17 @@ fc_status fc_distmap_query(fc_context* ctx, const fc_distmap* map, const float* 
  * distance 1000 (and cps/normals untouched) outside the box. */
 fc_status fc_distmap_eval(fc_context* ctx, const fc_distmap* map, const float* v, uint32_t n,
                           const fc_trans_info* info, float* cps, float* normals, float* distance);
+/* ColliderMeshMesh<gpu>::collide (collidermeshmesh.cuh:89-172): eval_distmap
+ * of body 1's model-frame vertices (device, 3 floats each) in body 0's map,
+ * keeping the vertices with distance < threshold (the reference uses
+ * 0.0015) as contacts, in vertex order: world contact point, unit normal
+ * and vertex index into cps/normals/vertexIds (device, capacity entries).
+ * *numContacts (host) receives the number of contacts, which may exceed
+ * capacity (only the first capacity are written).  Synchronises. */
+fc_status fc_mesh_mesh_contacts(fc_context* ctx, const fc_distmap* map, const float* v, uint32_t n,
+                                const fc_trans_info* info, float threshold, float* cps,
+                                float* normals, uint32_t* vertexIds, uint32_t capacity,
+                                uint32_t* numContacts);
 /* integrateSystem with the distance-map boundary on caller arrays (device
  * float4 pos/vel), using the context parameters. */
 fc_status fc_integrate_system_distmap(fc_context* ctx, const fc_distmap* map, float* pos,
@@ -488,6 +499,39 @@ fc_status fc_body_set_destroy(fc_body_set* set);
 fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrogate,
                                const float* margins, const float* pos, uint32_t n, float radius,
                                uint8_t* candidate, uint8_t* contact, uint32_t counts[3]);
+
+/* ------------------------------------------------------------------------- */
+/* Rigid bodies moved by the particle step (CollisionPipelineGPU).           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fc_coupling fc_coupling;
+/* initParticleSystem (fortrancppinterface/cppinterface.cpp:1262-1307): a
+ * particle system with one sphere per rigid body at its centre of mass
+ * (host, 3 doubles per body), at rest, with the coupling's parameters
+ * (globalDamping 1, gravity (0,0,-9.81), spring 2.75, damping 0.02, shear
+ * 0.1, attraction 0; the rest the ParticleSystem defaults).  The bodies'
+ * binary64 positions and velocities are kept on the host. */
+fc_status fc_coupling_create(fc_context* ctx, uint32_t numBodies, const double* com,
+                             fc_uint3 gridSize, fc_coupling** out);
+fc_status fc_coupling_destroy(fc_coupling* coupling);
+/* The coupling's particle system (for parameters, arrays, boundaries). */
+fc_system* fc_coupling_system(fc_coupling* coupling);
+/* Time step of fc_coupling_step callers that take it from the coupling
+ * (startcollisionpipeline_). */
+fc_status fc_coupling_set_time_step(fc_coupling* coupling, double dt);
+/* CollisionPipelineGPU::startPipeline (collision/collisionpipelinegpu.cpp:
+ * 34-61): subSteps updates of (float)dt / subSteps (the reference uses 10),
+ * then every sphere's position and velocity into its body. */
+fc_status fc_coupling_step(fc_coupling* coupling, double dt, uint32_t subSteps);
+/* A body's position and velocity (host, 3 doubles each; either may be
+ * NULL); set also moves or accelerates its sphere. */
+fc_status fc_coupling_get_body(const fc_coupling* coupling, uint32_t id, double pos[3],
+                               double vel[3]);
+fc_status fc_coupling_set_body(fc_coupling* coupling, uint32_t id, const double pos[3],
+                               const double vel[3]);
+/* The coupling the FEATFLOW entry points of fc_featflow.h act on (NULL
+ * unbinds; destroying the bound coupling unbinds it). */
+void fc_featflow_bind(fc_coupling* coupling);
 
 /* ------------------------------------------------------------------------- */
 /* Output files.                                                              */

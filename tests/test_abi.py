@@ -17,6 +17,8 @@ def declared_functions() -> list[str]:
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
         names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(fc_\w+)\s*\(", text, flags=re.M)
+        # the FEATFLOW entry points keep the Fortran names
+        names += re.findall(r"^\s*void\s+([a-z_0-9]+_)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
