@@ -79,10 +79,11 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
     if name == "collide":  # own pos/vel 32 B, index 4 B, sorted hash 4 B, newVel 16 B
         # (+ the positions scattered back to id order by a run's last step)
         return 56 * n + (16 * n if run else 0) + 8 * c_touch
-    if name == "collide_integrate":  # own pos/vel 32, index 4, hash 4; pos/vel 32, key 4, slot 4
-        return 80 * n + 8 * c_touch  # (the newVel store and reload are fused away)
-    if name == "reorder":  # hash+index 8 B, gather 32 B (+ slot map 4 B), write 32 B, tables
-        return 72 * n + (4 * n if run else 0) + 8 * c_occ
+    if name == "collide_integrate":  # own pos/vel 32, index 4, hash 4; pos/vel 32, key 4,
+        return 84 * n + 8 * c_touch  # slot 4, id of slot 4 (newVel store/reload fused away)
+    if name == "reorder":  # hash + sorted values 8 B, gather 32 B, write 32 B, tables; in a
+        # run the values are slots and the ids come through the slot's id (read 4 B, write 4 B)
+        return 72 * n + (8 * n if run else 0) + 8 * c_occ
     if name.startswith("sort_pass"):  # keys+values read and written once
         return 16 * n
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B

@@ -123,14 +123,17 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(
 // reorderDataAndFindCellStartD (particles_kernel.cu:119-182).  The
 // neighbour's hash comes from the previous lane through a cross-lane
 // shuffle; lane 0 of each wave reads it from memory.
-// With slotOf, particle id p's state lies at slot slotOf[p] of oldPos/oldVel
-// (the sorted-slot layout kept between the fused steps of a run).
+// vals are the sorted values: particle ids, or -- with idOfSlot, in the
+// sorted-slot layout kept between the fused steps of a run -- the slots of
+// oldPos/oldVel that hold each particle, whose ids idOfSlot gives; then the
+// ids are written to indexOut.
 __global__ __launch_bounds__(kBlock) void k_reorder(
     uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd,
     float4* __restrict__ sortedPos, float4* __restrict__ sortedVel,
-    const uint32_t* __restrict__ hash, const uint32_t* __restrict__ index,
-    const uint32_t* __restrict__ slotOf, const float4* __restrict__ oldPos,
-    const float4* __restrict__ oldVel, uint32_t n, uint32_t numCells) {
+    const uint32_t* __restrict__ hash, const uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ idOfSlot, uint32_t* __restrict__ indexOut,
+    const float4* __restrict__ oldPos, const float4* __restrict__ oldVel, uint32_t n,
+    uint32_t numCells) {
   uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   uint32_t h = (k < n) ? hash[k] : 0u;
   uint32_t hprev = __shfl_up(h, 1);
@@ -143,10 +146,10 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     if (k > 0 && hprev < numCells) cellEnd[hprev] = k;
   }
   if (k == n - 1 && h < numCells) cellEnd[h] = k + 1;
-  uint32_t src = index[k];
-  if (slotOf) src = slotOf[src];
+  const uint32_t src = vals[k];
   sortedPos[k] = oldPos[src];
   sortedVel[k] = oldVel[src];
+  if (idOfSlot) indexOut[k] = idOfSlot[src];
 }
 
 // Sum of the pair forces of cells [s, e) on particle k, in index order,
@@ -234,8 +237,9 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
 // radix sort's digit histogram of the hashes.  The integrated state is
 // written at the sorted slot k (pos[k], newVel[k], coalesced), the hash at
 // the particle's id (keysOut[id], so that the stable sort still orders ties
-// by id) and slotOf[id] = k for the next reorder.  Every value is the one
-// the separate stages would compute.
+// by id), slotOf[id] = k (the value the sort carries) and idOfSlot[k] = id
+// (for the next reorder).  Every value is the one the separate stages would
+// compute.
 // kPoly takes each sphere's radius from pos.w (a radius above
 // particleRadius raises DEVERR_RADIUS: the neighbour search and the contact
 // pre-filter are sized by particleRadius).
@@ -248,8 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-    uint32_t* devFlags, uint32_t* __restrict__ slotOf, const uint32_t* __restrict__ sortedHash,
-    v3f invCell) {
+    uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
+    const uint32_t* __restrict__ sortedHash, v3f invCell) {
   __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -281,6 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
       key = hash_of_q(P, cell_quotients(P, invCell, p));
       keysOut[orig] = key;
       slotOf[orig] = k;
+      idOfSlot[k] = orig;
     }
   }
   if (kFuse) {
@@ -344,10 +349,12 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const Bo
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
-                          uint32_t numCells, const uint32_t* slotOf) {
+                          uint32_t numCells, const uint32_t* idOfSlot, uint32_t* indexOut) {
   if (n == 0) return hipSuccess;
+  if (idOfSlot && !indexOut) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, cellStart, cellEnd,
-                     sortedPos, sortedVel, hash, index, slotOf, oldPos, oldVel, n, numCells);
+                     sortedPos, sortedVel, hash, index, idOfSlot, indexOut, oldPos, oldVel, n,
+                     numCells);
   return hipGetLastError();
 }
 
@@ -393,17 +400,18 @@ static void launch_v3(hipStream_t s, bool rows, dim3 g, dim3 b, const fc_sim_par
                       const float4* sortedVel, const uint32_t* index, const uint32_t* cellStart,
                       const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-                      uint32_t* devFlags, uint32_t* slotOf, const uint32_t* sortedHash) {
+                      uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
+                      const uint32_t* sortedHash) {
   if (rows)
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
-                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, sortedHash,
-                       inv_cell(P));
+                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
+                       sortedHash, inv_cell(P));
   else
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
-                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, sortedHash,
-                       inv_cell(P));
+                       dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
+                       sortedHash, inv_cell(P));
 }
 
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
@@ -426,11 +434,11 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
   if (poly)
     launch_v3<false, true>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                            index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                           nullptr, devFlags, nullptr, sortedHash);
+                           nullptr, devFlags, nullptr, nullptr, sortedHash);
   else
     launch_v3<false, false>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                             index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                            nullptr, devFlags, nullptr, sortedHash);
+                            nullptr, devFlags, nullptr, nullptr, sortedHash);
   return hipGetLastError();
 }
 
@@ -444,7 +452,7 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, const uint32_t* sortedHash) {
+                                uint32_t* slotOf, uint32_t* idOfSlot, const uint32_t* sortedHash) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
@@ -452,11 +460,11 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
   if (B.poly)
     launch_v3<true, true>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                           cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                          devFlags, slotOf, sortedHash);
+                          devFlags, slotOf, idOfSlot, sortedHash);
   else
     launch_v3<true, false>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                            cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                           devFlags, slotOf, sortedHash);
+                           devFlags, slotOf, idOfSlot, sortedHash);
   return hipGetLastError();
 }
 

@@ -159,11 +159,14 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const Bo
 hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
                                     float4* pos, float4* vel, float dt, uint32_t n);
 DistMapView distmap_view(const fc_distmap* m);
-// slotOf (optional): particle id p's state is at slot slotOf[p] of oldPos/oldVel.
+// index: the sorted values, particle ids; or, with idOfSlot, the slots of
+// oldPos/oldVel holding each particle (whose ids idOfSlot gives, written to
+// indexOut).
 hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd, float4* sortedPos,
                           float4* sortedVel, const uint32_t* hash, const uint32_t* index,
                           const float4* oldPos, const float4* oldVel, uint32_t n,
-                          uint32_t numCells, const uint32_t* slotOf = nullptr);
+                          uint32_t numCells, const uint32_t* idOfSlot = nullptr,
+                          uint32_t* indexOut = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
 // posOut (optional): also scatter the sorted positions to id order.
 // sortedHash (optional): the sorted hashes, which enable the row-range
@@ -184,7 +187,7 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, const uint32_t* sortedHash);
+                                uint32_t* slotOf, uint32_t* idOfSlot, const uint32_t* sortedHash);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).
@@ -223,7 +226,8 @@ struct fc_system {
   uint32_t* hash = nullptr;       // sorted hashes of the last update
   uint32_t* index = nullptr;      // sorted original indices of the last update
   uint32_t* keys = nullptr;       // unsorted hashes written by integrate+hash
-  uint32_t* slotOf = nullptr;     // multi-step runs: sorted slot of each particle id
+  uint32_t* slotOf = nullptr;     // multi-step runs: slot of each particle id (sort values)
+  uint32_t* idOfSlot = nullptr;   // multi-step runs: particle id held by each slot
   uint32_t* cellStart = nullptr;
   uint32_t* cellEnd = nullptr;
   // True when the cell tables hold exactly the cells listed in `hash`, so the

@@ -43,10 +43,12 @@ void free_system_buffers(fc_system* sys) {
   hipFree(sys->index);
   hipFree(sys->keys);
   hipFree(sys->slotOf);
+  hipFree(sys->idOfSlot);
   hipFree(sys->cellStart);
   hipFree(sys->cellEnd);
   sys->pos = sys->vel = sys->sortedPos = sys->sortedVel = nullptr;
-  sys->hash = sys->index = sys->keys = sys->slotOf = sys->cellStart = sys->cellEnd = nullptr;
+  sys->hash = sys->index = sys->keys = sys->slotOf = sys->idOfSlot = nullptr;
+  sys->cellStart = sys->cellEnd = nullptr;
 }
 
 void drop_graph(fc_system* sys) {
@@ -123,11 +125,13 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
 //   tail:   sort(+clear) reorder [collide]
 // which computes exactly what `steps` separate updates compute.
 // Between the fused steps the particle state stays in the previous sort's
-// order: the fused collide writes pos/vel at each particle's sorted slot
-// (coalesced) and records the slot in slotOf[id]; only the hash is written
-// at the id, so the sort input and its tie order are those of the separate
-// updates.  The next reorder gathers through slotOf, and the tail's collide
-// scatters the positions (with the velocities) back to id order.
+// order: the fused collide writes pos/vel at each particle's sorted slot k
+// (coalesced), the hash and slotOf[id] = k at the particle's id and
+// idOfSlot[k] = id.  The sort carries the slots as its values, in id order,
+// so its input and tie order are those of the separate updates; the next
+// reorder gathers pos/vel at the sorted slots (nearly in order) and writes
+// the ids through idOfSlot.  The tail's collide scatters the positions
+// (with the velocities) back to id order.
 int run_passes(const fc_sim_params& P) {
   const int p = radix_passes(key_bits_for_cells(P.numCells), false);
   return p < 2 ? 2 : p;
@@ -170,15 +174,20 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   const fc_sim_params& P = sys->params;
   const uint32_t n = sys->n;
   const int passes = run_passes(P);
-  hipError_t e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash,
-                                  sys->index, n, key_bits_for_cells(P.numCells), timer, true,
-                                  passes, clearInSort ? sys->hash : nullptr, sys->cellStart,
-                                  sys->cellEnd);
+  hipError_t e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys,
+                                  fromSlots ? sys->slotOf : nullptr, sys->hash,
+                                  fromSlots ? sys->slotOf : sys->index, n,
+                                  key_bits_for_cells(P.numCells), timer, true, passes,
+                                  clearInSort ? sys->hash : nullptr, sys->cellStart, sys->cellEnd);
   if (e != hipSuccess) return hip_error(e, "sort");
   if (timer) timer->begin(st, "reorder");
-  e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,
-                     sys->index, sys->pos, sys->vel, n, P.numCells,
-                     fromSlots ? sys->slotOf : nullptr);
+  if (fromSlots)
+    e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel,
+                       sys->hash, sys->slotOf, sys->pos, sys->vel, n, P.numCells, sys->idOfSlot,
+                       sys->index);
+  else
+    e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel,
+                       sys->hash, sys->index, sys->pos, sys->vel, n, P.numCells);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "reorder");
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
@@ -189,7 +198,8 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
-                             sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf, sys->hash);
+                             sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf,
+                             sys->idOfSlot, sys->hash);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
@@ -309,6 +319,7 @@ fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 grid
   if (e == hipSuccess) e = hipMalloc(&sys->index, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->keys, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->slotOf, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&sys->idOfSlot, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemset(sys->pos, 0, n * sizeof(float4));
   if (e == hipSuccess) e = hipMemset(sys->vel, 0, n * sizeof(float4));
   if (e != hipSuccess) {
