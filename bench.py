@@ -34,6 +34,7 @@ METRIC = "particle-steps/s at 1M spheres, % of HBM peak; SDF-MLP vs distance-map
 HBM_PEAK_GBS = 8000.0
 N = 1 << 20
 GRID = 256
+SORT_PASSES = 3  # 8-bit digits of the 24-bit cell hashes of a 256^3 grid
 RADIUS = 1.0 / 128.0
 DT = 0.5
 SEED = 1973
@@ -81,11 +82,13 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
         return 56 * n + (16 * n if run else 0) + 8 * c_touch
     if name == "collide_integrate":  # own pos/vel 32, index 4, hash 4; pos/vel 32, key 4,
         return 84 * n + 8 * c_touch  # slot 4, id of slot 4 (newVel store/reload fused away)
-    if name == "reorder":  # hash + sorted values 8 B, gather 32 B, write 32 B, tables; in a
-        # run the values are slots and the ids come through the slot's id (read 4 B, write 4 B)
-        return 72 * n + (8 * n if run else 0) + 8 * c_occ
-    if name.startswith("sort_pass"):  # keys+values read and written once
-        return 16 * n
+    if name == "cell_bounds":  # sorted hashes 4 B, cellStart/cellEnd per occupied cell
+        return 4 * n + 8 * c_occ
+    if name.startswith("sort_pass"):  # keys+values read and written once; the last pass
+        # also gathers pos/vel into sorted order (32 B read, 32 B written) and, in a run,
+        # writes the ids through the slot map (4 B read, 4 B written)
+        last = int(name[len("sort_pass"):]) == SORT_PASSES - 1
+        return 16 * n + ((64 * n + (8 * n if run else 0)) if last else 0)
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
         return 72 * n
     return None

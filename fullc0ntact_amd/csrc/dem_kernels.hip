@@ -152,6 +152,24 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
   if (idOfSlot) indexOut[k] = idOfSlot[src];
 }
 
+// The cell-table half of k_reorder (the gather done by the sort's last
+// pass).
+__global__ __launch_bounds__(kBlock) void k_cell_bounds(uint32_t* __restrict__ cellStart,
+                                                        uint32_t* __restrict__ cellEnd,
+                                                        const uint32_t* __restrict__ hash,
+                                                        uint32_t n, uint32_t numCells) {
+  uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t h = (k < n) ? hash[k] : 0u;
+  uint32_t hprev = __shfl_up(h, 1);
+  if (k >= n) return;
+  if ((threadIdx.x & 63u) == 0u && k > 0) hprev = hash[k - 1];
+  if (k == 0 || h != hprev) {
+    if (h < numCells) cellStart[h] = k;
+    if (k > 0 && hprev < numCells) cellEnd[hprev] = k;
+  }
+  if (k == n - 1 && h < numCells) cellEnd[h] = k + 1;
+}
+
 // Sum of the pair forces of cells [s, e) on particle k, in index order,
 // with a one-ahead prefetch of the next neighbour's position and velocity.
 __device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel, uint32_t k,
@@ -355,6 +373,14 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, cellStart, cellEnd,
                      sortedPos, sortedVel, hash, index, idOfSlot, indexOut, oldPos, oldVel, n,
                      numCells);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
+                              const uint32_t* hash, uint32_t n, uint32_t numCells) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, cellStart,
+                     cellEnd, hash, n, numCells);
   return hipGetLastError();
 }
 

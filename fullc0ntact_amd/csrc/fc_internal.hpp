@@ -106,12 +106,29 @@ int radix_passes(int bits, bool inPlace);
 // clearHash: when given, the first pass also resets to EMPTY every cell of
 // cellStart/cellEnd that starts a run in the sorted array clearHash (the
 // previous update's cells); requires a pass count >= 2 (minPasses).
+// Particle state gathered by the sort's last pass into sorted order: for
+// the output at position g with value v, sortedPos[g] = pos[v] and
+// sortedVel[g] = vel[v]; with idOfSlot (values are slots) also
+// indexOut[g] = idOfSlot[v].
+struct SortGather {
+  const float4* pos = nullptr;
+  const float4* vel = nullptr;
+  const uint32_t* idOfSlot = nullptr;
+  float4* sortedPos = nullptr;
+  float4* sortedVel = nullptr;
+  uint32_t* indexOut = nullptr;
+};
+
 hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
                             const uint32_t* keysIn, const uint32_t* valsIn,
                             uint32_t* keysOut, uint32_t* valsOut, uint32_t n, int bits,
                             StageTimer* timer, bool histogramReady = false, int minPasses = 1,
                             const uint32_t* clearHash = nullptr, uint32_t* cellStart = nullptr,
-                            uint32_t* cellEnd = nullptr);
+                            uint32_t* cellEnd = nullptr, const SortGather* gather = nullptr);
+// The cell-table half of reorderDataAndFindCellStart: cellStart/cellEnd of
+// the sorted hashes (the tables already hold EMPTY for every other cell).
+hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
+                              const uint32_t* hash, uint32_t n, uint32_t numCells);
 
 // ---------------------------------------------------------------------------
 // Per-stage timing with HIP events.

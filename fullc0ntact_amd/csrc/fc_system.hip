@@ -98,14 +98,20 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
                             ctx->devFlags);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "integrate_hash");
+  // The reorder's gather rides on the sort's last pass; its cell tables
+  // follow in their own launch.
+  SortGather ga;
+  ga.pos = sys->pos;
+  ga.vel = sys->vel;
+  ga.sortedPos = sys->sortedPos;
+  ga.sortedVel = sys->sortedVel;
   e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index, n,
-                       bits, timer, /*histogramReady=*/true);
+                       bits, timer, /*histogramReady=*/true, 1, nullptr, nullptr, nullptr, &ga);
   if (e != hipSuccess) return hip_error(e, "sort");
-  if (timer) timer->begin(st, "reorder");
-  e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel, sys->hash,
-                     sys->index, sys->pos, sys->vel, n, P.numCells);
+  if (timer) timer->begin(st, "cell_bounds");
+  e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
   if (timer) timer->end(st);
-  if (e != hipSuccess) return hip_error(e, "reorder");
+  if (e != hipSuccess) return hip_error(e, "cell bounds");
   if (timer) timer->begin(st, "collide");
   e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                      sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr,
@@ -174,22 +180,28 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   const fc_sim_params& P = sys->params;
   const uint32_t n = sys->n;
   const int passes = run_passes(P);
+  // The sort's last pass gathers the state into sorted order (at the
+  // sorted slots, writing the ids through idOfSlot, when fromSlots).
+  SortGather ga;
+  ga.pos = sys->pos;
+  ga.vel = sys->vel;
+  ga.sortedPos = sys->sortedPos;
+  ga.sortedVel = sys->sortedVel;
+  if (fromSlots) {
+    ga.idOfSlot = sys->idOfSlot;
+    ga.indexOut = sys->index;
+  }
   hipError_t e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys,
                                   fromSlots ? sys->slotOf : nullptr, sys->hash,
                                   fromSlots ? sys->slotOf : sys->index, n,
                                   key_bits_for_cells(P.numCells), timer, true, passes,
-                                  clearInSort ? sys->hash : nullptr, sys->cellStart, sys->cellEnd);
+                                  clearInSort ? sys->hash : nullptr, sys->cellStart, sys->cellEnd,
+                                  &ga);
   if (e != hipSuccess) return hip_error(e, "sort");
-  if (timer) timer->begin(st, "reorder");
-  if (fromSlots)
-    e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel,
-                       sys->hash, sys->slotOf, sys->pos, sys->vel, n, P.numCells, sys->idOfSlot,
-                       sys->index);
-  else
-    e = launch_reorder(st, sys->cellStart, sys->cellEnd, sys->sortedPos, sys->sortedVel,
-                       sys->hash, sys->index, sys->pos, sys->vel, n, P.numCells);
+  if (timer) timer->begin(st, "cell_bounds");
+  e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
   if (timer) timer->end(st);
-  if (e != hipSuccess) return hip_error(e, "reorder");
+  if (e != hipSuccess) return hip_error(e, "cell bounds");
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
