@@ -51,6 +51,13 @@ constexpr uint64_t kFlagMask = 3ull << 62;
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kGroup = RS_LOOKBACK_GROUP;  // tiles per look-back group
+// Tile counts up to which a pass is launched without the ticket counter:
+// one 1024-thread block per compute unit of an MI355X (256) is resident at
+// once.
+#ifndef FC_RS_UNTICKETED_TILES
+#define FC_RS_UNTICKETED_TILES 256
+#endif
+constexpr uint32_t kUnticketedTiles = FC_RS_UNTICKETED_TILES;
 #ifndef FC_RS_GROUP_WINDOW
 #define FC_RS_GROUP_WINDOW 4
 #endif
@@ -113,7 +120,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int pass,
     int passes, uint32_t* gHist, uint64_t* status, uint64_t* groupStatus, uint32_t* counters,
     uint32_t* devFlags, const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
-    uint32_t* __restrict__ cellEnd, SortGather ga) {
+    uint32_t* __restrict__ cellEnd, SortGather ga, int ticketed) {
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
   __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
@@ -128,7 +135,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   const uint32_t wave = threadIdx.x >> 6;
   const int shift = pass * RS_RADIX_BITS;
 
-  if (threadIdx.x == 0) {
+  // Tiles in launch order: from a ticket counter, or -- when every block of
+  // the launch can be resident at once -- the block index itself (a block
+  // then only waits for blocks that are running).
+  const uint32_t epochBase = counters[RSC_EPOCH];
+  if (ticketed && threadIdx.x == 0) {
     uint32_t t = atomicAdd(&counters[RSC_TILE0 + pass], 1u);
     if (t == numTiles - 1u) atomicExch(&counters[RSC_TILE0 + pass], 0u);
     s_tile = t;
@@ -140,8 +151,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     for (int i = threadIdx.x; i < RS_HIST_COPIES * RS_RADIX; i += RS_THREADS) h[i] = 0u;
   }
   __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t epoch = (counters[RSC_EPOCH] * 4u + (uint32_t)pass) & kEpochMask;
+  const uint32_t tile = ticketed ? s_tile : blockIdx.x;
+  const uint32_t epoch = (epochBase * 4u + (uint32_t)pass) & kEpochMask;
   const uint32_t tileBase = tile * RS_TILE;
   const uint32_t waveBase = tileBase + wave * kWaveKeys;
   if (clearHash != nullptr) {
@@ -398,7 +409,8 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
     hipLaunchKernelGGL(k_rs_scatter, dim3(numTiles), dim3(RS_THREADS), 0, stream, kin, vin, kout,
                        vout, n, p, passes, s.histogram, s.status, s.groupStatus, s.counters,
                        devFlags, p == 0 ? clearHash : nullptr, cellStart, cellEnd,
-                       (gather && p == passes - 1) ? *gather : SortGather{});
+                       (gather && p == passes - 1) ? *gather : SortGather{},
+                       numTiles > kUnticketedTiles ? 1 : 0);
     e = hipGetLastError();
     if (timer) timer->end(stream);
     if (e != hipSuccess) return e;
