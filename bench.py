@@ -256,6 +256,42 @@ def cpu_leg(target_seconds: float, threads: int) -> dict:
     return dict(value=N * k / el, steps=k, seconds=el)
 
 
+SMALL_N, SMALL_GRID, SMALL_RADIUS, SMALL_STEPS = 4096, 64, 1.0 / 64, 1000
+
+
+def small_case_leg(threads: int) -> dict:
+    """The 4096-sphere case (r = 1/64, 64^3 grid, 1000 steps of dt 0.5):
+    GPU graph-launched run, and the CPU restatement on `threads` threads and
+    on one."""
+    from fullc0ntact_amd import ParticleSystem, make_params
+    from oracle import oracle as orc
+
+    ps = ParticleSystem(SMALL_N, (SMALL_GRID,) * 3)
+    ps.m_params = make_params(SMALL_N, SMALL_GRID, SMALL_RADIUS)
+    ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
+    ps.run(DT, 20)  # warm-up (graph capture)
+    ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
+    ps.sync()
+    t0 = time.perf_counter()
+    ps.run(DT, SMALL_STEPS)
+    ps.sync()
+    gpu_s = time.perf_counter() - t0
+    ps.close()
+    out = {"particles": SMALL_N, "grid": [SMALL_GRID] * 3, "radius": SMALL_RADIUS,
+           "steps": SMALL_STEPS, "gpu_value": round(SMALL_N * SMALL_STEPS / gpu_s, 1)}
+    P = orc.make_params(SMALL_N, SMALL_GRID, SMALL_RADIUS)
+    for key, t in (("cpu_value", threads), ("cpu_value_1_core", 1)):
+        orc.set_threads(t)
+        pos, vel = orc.reset(SMALL_N, 1, SEED, SMALL_RADIUS, JITTER, VAMP)
+        st = orc.State(pos, vel, P.numCells)
+        t0 = time.perf_counter()
+        st.run(DT, SMALL_STEPS, P)
+        out[key] = round(SMALL_N * SMALL_STEPS / (time.perf_counter() - t0), 1)
+    orc.set_threads(1)
+    out["cpu_cores"] = threads
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,11 +346,18 @@ def main(argv=None) -> int:
         threads = max(1, min(16, os.cpu_count() or 1))
         log(f"[bench] CPU baseline with {threads} threads")
         c = cpu_leg(a.cpu_seconds, threads)
+        log("[bench] CPU baseline on one core")
+        c1 = cpu_leg(a.cpu_seconds / 2, 1)
+        log("[bench] 4096-sphere case, GPU and CPU")
+        small = small_case_leg(threads)
         cpu = {"value": round(c["value"], 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
                "sample": f"{c['steps']} updates of the same 1,048,576-sphere lattice after one "
                          f"warm-up update, {threads} OpenMP threads on {cpu_model()} "
-                         f"({c['seconds']:.1f} s)"}
+                         f"({c['seconds']:.1f} s)",
+               "value_1_core": round(c1["value"], 1),
+               "sample_1_core": f"{c1['steps']} updates on one thread ({c1['seconds']:.1f} s)",
+               "small_case": small}
 
     line = {
         "metric": METRIC,
