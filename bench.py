@@ -77,11 +77,12 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
     (the multi-step run, which keeps the particle state in sorted-slot order
     between fused steps) or "update" (single updates)."""
     run = path == "run"
-    if name == "collide":  # own pos/vel 32 B, index 4 B, sorted hash 4 B, newVel 16 B
-        # (+ the positions scattered back to id order by a run's last step)
-        return 56 * n + (16 * n if run else 0) + 8 * c_touch
-    if name == "collide_integrate":  # own pos/vel 32, index 4, hash 4; pos/vel 32, key 4,
-        return 84 * n + 8 * c_touch  # slot 4, id of slot 4 (newVel store/reload fused away)
+    if name == "collide":  # own pos/vel 32 B, index 4 B, newVel 16 B (the neighbours'
+        # hashes ride in sortedVel.w); + the positions scattered back to id order by a
+        # run's last step
+        return 52 * n + (16 * n if run else 0) + 8 * c_touch
+    if name == "collide_integrate":  # own pos/vel 32, index 4; pos/vel 32, key 4, slot 4,
+        return 80 * n + 8 * c_touch  # id of slot 4 (newVel store/reload fused away)
     if name == "cell_bounds":  # sorted hashes 4 B, cellStart/cellEnd per occupied cell
         return 4 * n + 8 * c_occ
     if name.startswith("sort_pass"):  # keys+values read and written once; the last pass

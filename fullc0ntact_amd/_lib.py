@@ -208,6 +208,7 @@ _SIGS = {
     "fc_sdf_train": (_st, [_vp, _vp, _vp, _u32, _u32, _u32, C.c_float, C.c_float, _vp]),
     "fc_sdf_eval": (_st, [_vp, _vp, _vp, _u32, _vp, C.c_int]),
     "fc_mfma_probe": (_st, [_vp, _vp, _vp, _vp]),
+    "fc_selftest_pair_math": (_st, [_vp, _vp]),
     "fc_body_set_create": (_st, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, _vp,
                                  C.POINTER(_vp)]),
     "fc_body_set_destroy": (_st, [_vp]),

@@ -61,6 +61,7 @@ constexpr int kListCap = FC_LIST_CAP;
 #endif
 constexpr int kRowsBatch = FC_ROWS_BATCH;
 constexpr int kPairBatch = FC_PAIR_BATCH;
+static_assert(kRowsBatch >= 1 && (uint32_t)kRowsBatch - 1u <= kSortedPosPad, "row padding");
 // Candidates whose loads are issued together in k_collide_v3.
 constexpr int kBatch = 4;
 
@@ -213,24 +214,28 @@ __device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteO
 }
 
 // Row-range variant of collide_one_v3 for grids with every dimension >= 4,
-// numCells <= 2^30 and the sorted hashes at hand.
+// numCells <= 2^30 and n < 2^28, on the arrays of a sort that gathered the
+// state (SortGather): sortedVel[j].w holds the bits of sphere j's sorted
+// hash, and sortedPos has kRowsBatch - 1 readable entries past the last
+// sphere.
 //
 // The three cells of a neighbour row (dx = -1, 0, 1) have consecutive
 // hashes unless the row wraps in x, so their particles are one contiguous
 // range [lo, lo + len) of the sorted order, visited in the cell order of
 // the 27-cell loop.  (A wrapping row is still one range when one of its two
 // hash runs is empty; a lane with a row of two non-empty runs takes the
-// direct loop.)
+// direct loop.)  Cells out of contact reach are cut from the ranges first.
 //
 // Phase A walks the three rows of each z-plane as one flattened sequence
-// and keeps the indices of candidates that pass a pre-filter in a per-lane
-// list in LDS.  The pre-filter evaluates d2 with fused multiply-adds and
-// compares against contactD2 raised by 2^-18 relative, which keeps every
-// candidate with (reference-rounded) d2 < contactD2; the extra near-misses
-// get the exact contact test in phase B and add +0.  Phase B walks the list
-// and detects a change of cell by a change of the neighbour's sorted hash
-// (all 27 cells are distinct when every grid dimension is >= 3), adding
-// each cell's partial to the total in cell order.  As in collide_one_v3
+// and keeps the byte offsets of candidates that pass a pre-filter in a
+// per-lane list in LDS.  The pre-filter evaluates d2 with fused
+// multiply-adds and compares against contactD2 raised by 2^-18 relative,
+// which keeps every candidate with (reference-rounded) d2 < contactD2; the
+// extra near-misses get the exact contact test in phase B and add +0.
+// Phase B walks the list and detects a change of cell by a change of the
+// neighbour's sorted hash (all 27 cells are distinct when every grid
+// dimension is >= 3), adding each cell's partial to the total in cell
+// order.  As in collide_one_v3
 // every dropped or non-touching candidate contributed exactly +0, so the
 // result is bit-identical to the 27-cell loop.
 template <bool kPoly, int BS>
@@ -238,7 +243,6 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
                                                 float contactD2, uint32_t k, v3f pos, v3f vel,
                                                 float rA, const float4* __restrict__ sortedPos,
                                                 const float4* __restrict__ sortedVel,
-                                                const uint32_t* __restrict__ sortedHash,
                                                 const uint32_t* __restrict__ cellStart,
                                                 const uint32_t* __restrict__ cellEnd, uint32_t n,
                                                 uint32_t (*s_list)[BS]) {
@@ -264,18 +268,50 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
     e[3 * r + 1] = load_off(cellEnd, o1);
     e[3 * r + 2] = load_off(cellEnd, o2);
   }
+
+  // ---- reachable cells ----
+  // Lower bounds on the squared distance from this sphere to the boxes of
+  // its lower and upper neighbour cells along each axis: the fraction of its
+  // cell quotient below / above it, shrunk by a margin (2^-20 (|q| + 1)
+  // cells) that covers the rounding of both spheres' quotients, times the
+  // cell size.  A sphere hashed to a neighbour cell lies in that box or, when
+  // the index wrapped, at least G - 2 >= 2 cells further along the axis, so
+  // no sphere of a cell whose bound reaches the pre-filter threshold (with a
+  // 2^-16 relative margin for the rounding of d2) can touch this one, and the
+  // cell is cut from its row's range.  The cut cells are always at the ends
+  // of a row, which stays one range.  A NaN bound keeps the cell.
+  const float thrA = fmaxf(contactD2 * (1.0f + 0x1p-18f), contactD2 + 0x1p-140f);
+  const float thrCut = thrA * (1.0f + 0x1p-16f);
+  v3f lo2, hi2;
+  {
+    const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+    const float ux = fmaf(fabsf(q.x), 0x1p-20f, 0x1p-20f);
+    const float uy = fmaf(fabsf(q.y), 0x1p-20f, 0x1p-20f);
+    const float uz = fmaf(fabsf(q.z), 0x1p-20f, 0x1p-20f);
+    const float lx = fmaxf(fx - ux, 0.0f) * P.cellSize.x, hx = fmaxf(1.0f - fx - ux, 0.0f) * P.cellSize.x;
+    const float ly = fmaxf(fy - uy, 0.0f) * P.cellSize.y, hy = fmaxf(1.0f - fy - uy, 0.0f) * P.cellSize.y;
+    const float lz = fmaxf(fz - uz, 0.0f) * P.cellSize.z, hz = fmaxf(1.0f - fz - uz, 0.0f) * P.cellSize.z;
+    lo2 = make_v3(lx * lx, ly * ly, lz * lz);
+    hi2 = make_v3(hx * hx, hy * hy, hz * hz);
+  }
+
   const bool wrapLo = xc == 0u, wrapHi = xc == mx;
   uint32_t lo[9], len[9];
   bool split = false;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
-    const bool n0 = s[3 * r] != CELL_EMPTY;
-    const bool n1 = s[3 * r + 1] != CELL_EMPTY;
-    const bool n2 = s[3 * r + 2] != CELL_EMPTY;
+    const int dy = r % 3 - 1, dz = r / 3 - 1;
+    const float gyz = (dy < 0 ? lo2.y : (dy > 0 ? hi2.y : 0.0f)) +
+                      (dz < 0 ? lo2.z : (dz > 0 ? hi2.z : 0.0f));
+    const bool n0 = s[3 * r] != CELL_EMPTY && !(gyz + lo2.x >= thrCut);
+    const bool n1 = s[3 * r + 1] != CELL_EMPTY && !(gyz >= thrCut);
+    const bool n2 = s[3 * r + 2] != CELL_EMPTY && !(gyz + hi2.x >= thrCut);
     const uint32_t e0 = e[3 * r] > n ? n : e[3 * r];
     const uint32_t e1 = e[3 * r + 1] > n ? n : e[3 * r + 1];
     const uint32_t e2 = e[3 * r + 2] > n ? n : e[3 * r + 2];
-    const uint32_t a = n0 ? s[3 * r] : (n1 ? s[3 * r + 1] : s[3 * r + 2]);
+    // An empty row starts at 0, so that the padding slots of a plane whose
+    // last row is empty stay in the array.
+    const uint32_t a = n0 ? s[3 * r] : (n1 ? s[3 * r + 1] : (n2 ? s[3 * r + 2] : 0u));
     const uint32_t b = n2 ? e2 : (n1 ? e1 : e0);
     lo[r] = a;
     len[r] = (n0 || n1 || n2) && b > a ? b - a : 0u;
@@ -283,33 +319,37 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
   }
 
   // ---- phase A: candidates within the contact pre-filter ----
-  const float thrA = fmaxf(contactD2 * (1.0f + 0x1p-18f), contactD2 + 0x1p-140f);
-  const uint32_t last = n - 1u;
+  // The three rows of a z-plane are walked as one flattened sequence of
+  // slots, kRowsBatch per iteration with their loads in flight together.
+  // Slot t of the plane is sphere t + q(t), q stepping at the row
+  // boundaries; the list keeps the candidates' byte offsets.  A slot
+  // past the plane's end reads at most kRowsBatch - 1 entries past the last
+  // row's range (inside the array's padding) and its test is masked off.
+  // Only the centre plane holds the sphere itself.
+  const uint32_t k16 = k << 4;
   uint32_t cnt = 0;
 #pragma unroll
   for (int pl = 0; pl < 3; ++pl) {
     const uint32_t b1 = len[3 * pl], b2 = b1 + len[3 * pl + 1], tot = b2 + len[3 * pl + 2];
-    const uint32_t q0 = lo[3 * pl], q1 = lo[3 * pl + 1] - b1, q2 = lo[3 * pl + 2] - b2;
-    // kRowsBatch candidates per iteration; the index of a slot past the
-    // plane's last candidate is clamped into the arrays and its test is
-    // masked off, so the loads of one iteration are all in flight at once.
+    const uint32_t q0 = lo[3 * pl], q1 = lo[3 * pl + 1] - b1,
+                   q2 = lo[3 * pl + 2] - b2;  // modulo 2^32
 #pragma unroll 1
     for (uint32_t t = 0; t < tot; t += kRowsBatch) {
-      uint32_t jj[kRowsBatch];
+      uint32_t off[kRowsBatch];
       float4 qq[kRowsBatch];
 #pragma unroll
       for (int u = 0; u < kRowsBatch; ++u) {
         const uint32_t tu = t + (uint32_t)u;
-        const uint32_t j = tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2));
-        jj[u] = j < last ? j : last;
-        qq[u] = load_off(sortedPos, jj[u] << 4);
+        off[u] = (tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2))) << 4;
+        qq[u] = load_off(sortedPos, off[u]);
       }
 #pragma unroll
       for (int u = 0; u < kRowsBatch; ++u) {
         const float ex = qq[u].x - pos.x, ey = qq[u].y - pos.y, ez = qq[u].z - pos.z;
         const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = jj[u];
-        cnt += (d2 < thrA && jj[u] != k && (u == 0 || t + (uint32_t)u < tot)) ? 1u : 0u;
+        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = off[u];
+        const bool in = (u == 0 || t + (uint32_t)u < tot) && (pl != 1 || off[u] != k16);
+        cnt += (d2 < thrA && in) ? 1u : 0u;
       }
     }
   }
@@ -317,33 +357,33 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
   v3f force = make_v3(0.0f, 0.0f, 0.0f);
   if (!split && cnt <= (uint32_t)kListCap) {
     // ---- phase B ----
-    // kPairBatch list entries per iteration (entries past cnt load a
-    // clamped index and are masked off).
+    // kPairBatch list entries per iteration (entries past cnt load the
+    // sphere itself and are masked off).
     v3f cf = make_v3(0.0f, 0.0f, 0.0f);
     uint32_t curHash = 0xffffffffu;  // no cell has this hash (numCells <= 2^30)
 #pragma unroll 1
     for (uint32_t i = 0; i < cnt; i += kPairBatch) {
       float4 qp[kPairBatch], qv[kPairBatch];
-      uint32_t hh[kPairBatch];
 #pragma unroll
       for (int u = 0; u < kPairBatch; ++u) {
         const uint32_t iu = i + (uint32_t)u;
-        uint32_t j = s_list[iu < (uint32_t)kListCap ? iu : (uint32_t)kListCap][threadIdx.x];
-        j = (u == 0 || iu < cnt) && j < last ? j : last;
-        qp[u] = load_off(sortedPos, j << 4);
-        qv[u] = load_off(sortedVel, j << 4);
-        hh[u] = load_off(sortedHash, j << 2);
+        uint32_t off = s_list[iu < (uint32_t)kListCap ? iu : (uint32_t)kListCap][threadIdx.x];
+        off = (u == 0 || iu < cnt) ? off : k16;  // a stale entry may lie in the padding
+        qp[u] = load_off(sortedPos, off);
+        qv[u] = load_off(sortedVel, off);  // .w holds the sorted hash
       }
 #pragma unroll
       for (int u = 0; u < kPairBatch; ++u) {
         if (u == 0 || i + (uint32_t)u < cnt) {
-          if (hh[u] != curHash) {
+          const uint32_t hu = __float_as_uint(qv[u].w);
+          if (hu != curHash) {
             force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
             cf = make_v3(0.0f, 0.0f, 0.0f);
-            curHash = hh[u];
+            curHash = hu;
           }
-          v3f f = collide_spheres(pos, xyz(qp[u]), vel, xyz(qv[u]), rA, kPoly ? qp[u].w : rA,
-                                  P.spring, P.damping, P.shear, P.attraction);
+          v3f f = collide_spheres_fast(pos, xyz(qp[u]), vel, xyz(qv[u]), rA,
+                                       kPoly ? qp[u].w : rA, P.spring, P.damping, P.shear,
+                                       P.attraction);
           cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
         }
       }

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
     uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
-    const uint32_t* __restrict__ sortedHash, v3f invCell) {
+    v3f invCell) {
   __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const v3f nv =
         kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, xyz(p4),
                                                 xyz(sortedVel[k]), rA, sortedPos, sortedVel,
-                                                sortedHash, cellStart, cellEnd, n, s_list)
+                                                cellStart, cellEnd, n, s_list)
               : collide_one_v3<kPoly, kBlock>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
                                               sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
     const uint32_t orig = index[k];
@@ -396,10 +396,12 @@ float contact_d2_threshold(float R) {
   return t;
 }
 
-// The row-range walk needs the sorted hashes, 27 distinct neighbour cells
-// (every grid dimension >= 4 here) and cell byte offsets below 2^32.
-static bool rows_walk_ok(const fc_sim_params& P, const uint32_t* sortedHash) {
-  return sortedHash != nullptr && P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
+// The row-range walk needs arrays gathered by the sort (the sorted hash in
+// sortedVel.w, kSortedPosPad entries of padding after sortedPos: a system's
+// own arrays), 27 distinct neighbour cells (every grid dimension >= 4 here)
+// and cell byte offsets below 2^32.
+static bool rows_walk_ok(const fc_sim_params& P, bool gathered) {
+  return gathered && P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
          P.numCells <= (1u << 30);
 }
 
@@ -426,25 +428,24 @@ static void launch_v3(hipStream_t s, bool rows, dim3 g, dim3 b, const fc_sim_par
                       const float4* sortedVel, const uint32_t* index, const uint32_t* cellStart,
                       const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-                      uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
-                      const uint32_t* sortedHash) {
+                      uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot) {
   if (rows)
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       sortedHash, inv_cell(P));
+                       inv_cell(P));
   else
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       sortedHash, inv_cell(P));
+                       inv_cell(P));
 }
 
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut,
-                          const uint32_t* sortedHash) {
+                          bool gathered) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
@@ -456,15 +457,15 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
                        cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
     return hipGetLastError();
   }
-  const bool rows = rows_walk_ok(P, sortedHash);
+  const bool rows = rows_walk_ok(P, gathered);
   if (poly)
     launch_v3<false, true>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                            index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                           nullptr, devFlags, nullptr, nullptr, sortedHash);
+                           nullptr, devFlags, nullptr, nullptr);
   else
     launch_v3<false, false>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                             index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                            nullptr, devFlags, nullptr, nullptr, sortedHash);
+                            nullptr, devFlags, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -478,19 +479,19 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, uint32_t* idOfSlot, const uint32_t* sortedHash) {
+                                uint32_t* slotOf, uint32_t* idOfSlot, bool gathered) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
-  const bool rows = rows_walk_ok(P, sortedHash);
+  const bool rows = rows_walk_ok(P, gathered);
   if (B.poly)
     launch_v3<true, true>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                           cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                          devFlags, slotOf, idOfSlot, sortedHash);
+                          devFlags, slotOf, idOfSlot);
   else
     launch_v3<true, false>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                            cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                           devFlags, slotOf, idOfSlot, sortedHash);
+                           devFlags, slotOf, idOfSlot);
   return hipGetLastError();
 }
 

@@ -91,6 +91,27 @@ __device__ __forceinline__ uint32_t hash_of_q(const fc_sim_params& P, v3f q) {
 //              u = vB - vA;  ut = u - (u.n) n
 //              f = (-k (R - dist)) n + c u + s ut + a d
 //   else 0.
+// contact_force is the touching branch, given dist and inv = 1/dist.
+__device__ __forceinline__ v3f contact_force(v3f rel, float dist, float inv, float collideDist,
+                                             v3f vA, v3f vB, float spring, float damping,
+                                             float shear, float attraction) {
+  v3f nrm = make_v3(rel.x * inv, rel.y * inv, rel.z * inv);
+  v3f rv = make_v3(vB.x - vA.x, vB.y - vA.y, vB.z - vA.z);
+  float rvn = dot3(rv, nrm);
+  v3f tv = make_v3(rv.x - nrm.x * rvn, rv.y - nrm.y * rvn, rv.z - nrm.z * rvn);
+  float s = -spring * (collideDist - dist);
+  v3f f = make_v3(nrm.x * s, nrm.y * s, nrm.z * s);
+  f = make_v3(f.x + rv.x * damping, f.y + rv.y * damping, f.z + rv.z * damping);
+  f = make_v3(f.x + tv.x * shear, f.y + tv.y * shear, f.z + tv.z * shear);
+  // With attraction == 0 the term is +-0 (rel is finite here), which can
+  // only turn a -0 component into +0.  Every caller adds the result to a
+  // sum that starts at +0 and therefore never holds -0, where x + (-0)
+  // and x + (+0) are equal, so the term is skipped exactly.
+  if (__builtin_expect(attraction != 0.0f, 0))
+    f = make_v3(f.x + rel.x * attraction, f.y + rel.y * attraction, f.z + rel.z * attraction);
+  return f;
+}
+
 __device__ __forceinline__ v3f collide_spheres(v3f pA, v3f pB, v3f vA, v3f vB, float rA, float rB,
                                                float spring, float damping, float shear,
                                                float attraction) {
@@ -98,22 +119,72 @@ __device__ __forceinline__ v3f collide_spheres(v3f pA, v3f pB, v3f vA, v3f vB, f
   float dist = sqrtf(dot3(rel, rel));
   float collideDist = rA + rB;
   v3f f = make_v3(0.0f, 0.0f, 0.0f);
+  if (dist < collideDist)
+    f = contact_force(rel, dist, 1.0f / dist, collideDist, vA, vB, spring, damping, shear,
+                      attraction);
+  return f;
+}
+
+// Correctly rounded square root for x in [2^-96, 2^96].  v_sqrt_f32 is
+// within 1 ulp of the root, so the rounded root is s or one of its
+// neighbours sDn, sUp.  x <= sDn s (the sign of one FMA residual, which is
+// exact in sign) puts the root at or below the geometric mean of sDn and s;
+// between that mean and the midpoint of sDn and s lies no root of a float
+// (their squares differ by a quarter of ulp(s)^2, less than the spacing of
+// the floats there), so the root rounds to sDn.  x > sUp s rounds up in the
+// same way.  This is also the compiler's own expansion of sqrtf, without
+// its rescaling of tiny and pass-through of zero and infinite arguments,
+// which this range never needs.
+__device__ __forceinline__ float sqrt_rn_mid(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sDn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sUp = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = fmaf(-sDn, s, x) <= 0.0f ? sDn : s;
+  r = fmaf(-sUp, s, x) > 0.0f ? sUp : r;
+  return r;
+}
+
+// Correctly rounded 1 / b for b in [2^-48, 2^48] whose significand is not
+// all ones: v_rcp_f32 is within 1 ulp of 1/b, and one Newton step with an
+// FMA residual, y + y (1 - b y), rounds it correctly (Markstein, IBM J. Res.
+// Dev. 34(1):111-119, 1990, whose one exception is the all-ones
+// significand).  fc_selftest_pair_math checks sqrt_rn_mid and rcp_rn_mid
+// against the correctly rounded operations on every argument of their
+// ranges.
+__device__ __forceinline__ float rcp_rn_mid(float b) {
+  const float y = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, y, 1.0f);
+  return fmaf(e, y, y);
+}
+
+__device__ __forceinline__ bool rcp_rn_mid_ok(float b) {
+  return b >= 0x1p-48f && b <= 0x1p48f && (__float_as_uint(b) & 0x7fffffu) != 0x7fffffu;
+}
+
+// collide_spheres, bit for bit, with the square root and the reciprocal
+// taken through sqrt_rn_mid / rcp_rn_mid when every active lane's argument
+// is in their range.  The choice is a wave-uniform branch, so the general
+// sequences cost nothing while no lane needs them.
+__device__ __forceinline__ v3f collide_spheres_fast(v3f pA, v3f pB, v3f vA, v3f vB, float rA,
+                                                    float rB, float spring, float damping,
+                                                    float shear, float attraction) {
+  v3f rel = make_v3(pB.x - pA.x, pB.y - pA.y, pB.z - pA.z);
+  const float d2 = dot3(rel, rel);
+  const bool mid = d2 >= 0x1p-96f && d2 <= 0x1p96f;
+  float dist;
+  if (__builtin_amdgcn_ballot_w64(!mid) == 0u)
+    dist = sqrt_rn_mid(d2);
+  else
+    dist = sqrtf(d2);
+  float collideDist = rA + rB;
+  v3f f = make_v3(0.0f, 0.0f, 0.0f);
   if (dist < collideDist) {
-    float inv = 1.0f / dist;
-    v3f nrm = make_v3(rel.x * inv, rel.y * inv, rel.z * inv);
-    v3f rv = make_v3(vB.x - vA.x, vB.y - vA.y, vB.z - vA.z);
-    float rvn = dot3(rv, nrm);
-    v3f tv = make_v3(rv.x - nrm.x * rvn, rv.y - nrm.y * rvn, rv.z - nrm.z * rvn);
-    float s = -spring * (collideDist - dist);
-    f = make_v3(nrm.x * s, nrm.y * s, nrm.z * s);
-    f = make_v3(f.x + rv.x * damping, f.y + rv.y * damping, f.z + rv.z * damping);
-    f = make_v3(f.x + tv.x * shear, f.y + tv.y * shear, f.z + tv.z * shear);
-    // With attraction == 0 the term is +-0 (rel is finite here), which can
-    // only turn a -0 component into +0.  Every caller adds the result to a
-    // sum that starts at +0 and therefore never holds -0, where x + (-0)
-    // and x + (+0) are equal, so the term is skipped exactly.
-    if (attraction != 0.0f)
-      f = make_v3(f.x + rel.x * attraction, f.y + rel.y * attraction, f.z + rel.z * attraction);
+    float inv;
+    if (__builtin_amdgcn_ballot_w64(!rcp_rn_mid_ok(dist)) == 0u)
+      inv = rcp_rn_mid(dist);
+    else
+      inv = 1.0f / dist;
+    f = contact_force(rel, dist, inv, collideDist, vA, vB, spring, damping, shear, attraction);
   }
   return f;
 }

@@ -43,6 +43,9 @@ enum : uint32_t {
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
+// Entries allocated past the last sphere in a system's sortedPos (read, and
+// discarded, by the row-range collide's batched loads).
+constexpr uint32_t kSortedPosPad = 8;
 
 // ---------------------------------------------------------------------------
 // Radix sort (radix_sort.hip).
@@ -107,9 +110,9 @@ int radix_passes(int bits, bool inPlace);
 // cellStart/cellEnd that starts a run in the sorted array clearHash (the
 // previous update's cells); requires a pass count >= 2 (minPasses).
 // Particle state gathered by the sort's last pass into sorted order: for
-// the output at position g with value v, sortedPos[g] = pos[v] and
-// sortedVel[g] = vel[v]; with idOfSlot (values are slots) also
-// indexOut[g] = idOfSlot[v].
+// the output at position g with key h and value v, sortedPos[g] = pos[v]
+// and sortedVel[g] = (vel[v].xyz, bits of h); with idOfSlot (values are
+// slots) also indexOut[g] = idOfSlot[v].
 struct SortGather {
   const float4* pos = nullptr;
   const float4* vel = nullptr;
@@ -186,13 +189,14 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
                           uint32_t* indexOut = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
 // posOut (optional): also scatter the sorted positions to id order.
-// sortedHash (optional): the sorted hashes, which enable the row-range
-// neighbour walk.
+// gathered: sortedPos/sortedVel were written by a sort's SortGather into a
+// system's arrays (sortedVel.w holds the sorted hash, sortedPos carries
+// kSortedPosPad entries of padding), which enables the row-range walk.
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr,
-                          const uint32_t* sortedHash = nullptr);
+                          bool gathered = false);
 
 float contact_d2_threshold(float R);
 // collide + the next update's integrate/hash/histogram in one launch (see
@@ -204,7 +208,7 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, uint32_t* idOfSlot, const uint32_t* sortedHash);
+                                uint32_t* slotOf, uint32_t* idOfSlot, bool gathered);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).

@@ -115,7 +115,7 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   if (timer) timer->begin(st, "collide");
   e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                      sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr,
-                     sys->hash);
+                     /*gathered=*/true);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   sys->tablesTrackHash = true;
@@ -206,12 +206,12 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                        sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n,
-                       fromSlots ? sys->pos : nullptr, sys->hash);
+                       fromSlots ? sys->pos : nullptr, /*gathered=*/true);
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
                              sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf,
-                             sys->idOfSlot, sys->hash);
+                             sys->idOfSlot, /*gathered=*/true);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
@@ -325,7 +325,7 @@ fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 grid
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc(&sys->pos, n * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&sys->vel, n * sizeof(float4));
-  if (e == hipSuccess) e = hipMalloc(&sys->sortedPos, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->sortedPos, (n + kSortedPosPad) * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&sys->sortedVel, n * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&sys->hash, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->index, n * sizeof(uint32_t));

@@ -331,7 +331,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         valsOut[g] = v;
         if (ga.pos && v < n) {  // v < n: values index the particle arrays
           ga.sortedPos[g] = ga.pos[v];
-          ga.sortedVel[g] = ga.vel[v];
+          const float4 vv = ga.vel[v];
+          ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
           if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
         }
       }

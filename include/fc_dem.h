@@ -578,6 +578,16 @@ fc_status fc_copy_probe(fc_context* ctx, void* dst, const void* src, size_t byte
  * it: d = a b for a 16x32 and b 32x16 row-major bf16 matrices, d 16x16
  * row-major f32 (device arrays).  No counterpart in the reference. */
 fc_status fc_mfma_probe(fc_context* ctx, const uint16_t* a, const uint16_t* b, float* d);
+/* No counterpart in the reference: checks, on every float of their ranges,
+ * the correctly rounded square root (arguments in [2^-96, 2^96]) and
+ * reciprocal (arguments in [2^-48, 2^48]) that the contact loop uses in
+ * place of the general sequences, against sqrtf and 1.0f / x on the
+ * device.  out[0] = square-root mismatches, out[1] = reciprocal mismatches
+ * (arguments with an all-ones significand excluded; the contact loop takes
+ * the general division for those), out[2] = reciprocal mismatches at
+ * all-ones significands, out[3] / out[4] = bits of the smallest mismatching
+ * square-root / reciprocal argument (0xffffffff if none).  Synchronises. */
+fc_status fc_selftest_pair_math(fc_context* ctx, uint64_t out[5]);
 
 #ifdef __cplusplus
 }
