@@ -254,9 +254,10 @@ struct fc_system {
   // True when the cell tables hold exactly the cells listed in `hash`, so the
   // next update may clear only those instead of the whole table.
   bool tablesTrackHash = false;
-  // Captured steady-state update (graph-launched by fc_system_run).
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graphExec = nullptr;
+  // Captured steady-state steps, graph-launched by fc_system_run: [0] one
+  // step, [1] kGraphSteps steps back to back (fewer launches per step).
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graphExec[2] = {nullptr, nullptr};
   float graphDt = 0.0f;
   fc_sim_params graphParams{};
   // Boundary of the integrate stage.

@@ -11,6 +11,13 @@ namespace fc {
 
 namespace {
 
+// Steps per launch of the multi-step graph of a run (the kernel boundaries
+// inside a graph cost less than the boundary between two graph launches).
+#ifndef FC_GRAPH_STEPS
+#define FC_GRAPH_STEPS 8
+#endif
+constexpr uint32_t kGraphSteps = FC_GRAPH_STEPS;
+
 // SplitMix64 (Steele, Lea, Flood 2014).  Portable across compilers and
 // hosts, unlike srand/rand (particleSystem.cpp:375-383).
 struct SplitMix64 {
@@ -52,10 +59,12 @@ void free_system_buffers(fc_system* sys) {
 }
 
 void drop_graph(fc_system* sys) {
-  if (sys->graphExec) hipGraphExecDestroy(sys->graphExec);
-  if (sys->graph) hipGraphDestroy(sys->graph);
-  sys->graphExec = nullptr;
-  sys->graph = nullptr;
+  for (int i = 0; i < 2; ++i) {
+    if (sys->graphExec[i]) hipGraphExecDestroy(sys->graphExec[i]);
+    if (sys->graph[i]) hipGraphDestroy(sys->graph[i]);
+    sys->graphExec[i] = nullptr;
+    sys->graph[i] = nullptr;
+  }
 }
 
 fc_status alloc_tables(fc_system* sys, uint32_t numCells) {
@@ -217,12 +226,14 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   return FC_SUCCESS;
 }
 
-fc_status capture_graph(fc_system* sys, float dt) {
-  drop_graph(sys);
+// Captures `steps` steady-state run bodies back to back into graph[slot].
+fc_status capture_steps(fc_system* sys, float dt, int slot, uint32_t steps) {
   hipStream_t st = sys->ctx->stream;
   FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  fc_status s = enqueue_run_body(sys, dt, /*clearInSort=*/true, /*last=*/false,
-                                 /*fromSlots=*/true, nullptr);
+  fc_status s = FC_SUCCESS;
+  for (uint32_t i = 0; i < steps && s == FC_SUCCESS; ++i)
+    s = enqueue_run_body(sys, dt, /*clearInSort=*/true, /*last=*/false, /*fromSlots=*/true,
+                         nullptr);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (s != FC_SUCCESS) {
@@ -230,11 +241,19 @@ fc_status capture_graph(fc_system* sys, float dt) {
     return s;
   }
   if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
-  sys->graph = g;
-  e = hipGraphInstantiate(&sys->graphExec, g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
+  sys->graph[slot] = g;
+  e = hipGraphInstantiate(&sys->graphExec[slot], g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+  return FC_SUCCESS;
+}
+
+fc_status capture_graph(fc_system* sys, float dt) {
+  drop_graph(sys);
+  fc_status s = capture_steps(sys, dt, 0, 1);
+  if (s == FC_SUCCESS) s = capture_steps(sys, dt, 1, kGraphSteps);
+  if (s != FC_SUCCESS) {
     drop_graph(sys);
-    return hip_error(e, "hipGraphInstantiate");
+    return s;
   }
   sys->graphDt = dt;
   sys->graphParams = sys->params;
@@ -448,12 +467,14 @@ fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
       for (uint32_t i = 0; i < middle; ++i)
         FC_TRY(enqueue_run_body(sys, deltaTime, true, false, true, timer));
     } else {
-      if (!sys->graphExec || sys->graphDt != deltaTime ||
+      if (!sys->graphExec[0] || sys->graphDt != deltaTime ||
           std::memcmp(&sys->graphParams, &sys->params, sizeof(fc_sim_params)) != 0) {
         FC_TRY(capture_graph(sys, deltaTime));
       }
-      for (uint32_t i = 0; i < middle; ++i)
-        FC_HIP_TRY(hipGraphLaunch(sys->graphExec, ctx->stream));
+      uint32_t i = 0;
+      for (; i + kGraphSteps <= middle; i += kGraphSteps)
+        FC_HIP_TRY(hipGraphLaunch(sys->graphExec[1], ctx->stream));
+      for (; i < middle; ++i) FC_HIP_TRY(hipGraphLaunch(sys->graphExec[0], ctx->stream));
     }
   }
   FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/true, /*last=*/true,
