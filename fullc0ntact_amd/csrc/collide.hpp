@@ -235,9 +235,9 @@ __device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteO
 // Phase B walks the list and detects a change of cell by a change of the
 // neighbour's sorted hash (all 27 cells are distinct when every grid
 // dimension is >= 3), adding each cell's partial to the total in cell
-// order.  As in collide_one_v3
-// every dropped or non-touching candidate contributed exactly +0, so the
-// result is bit-identical to the 27-cell loop.
+// order.  As in collide_one_v3 every dropped or non-touching candidate
+// contributed exactly +0, so the result is bit-identical to the 27-cell
+// loop.
 template <bool kPoly, int BS>
 __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invCell,
                                                 float contactD2, uint32_t k, v3f pos, v3f vel,
