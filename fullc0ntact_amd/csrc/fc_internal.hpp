@@ -128,6 +128,17 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
                             StageTimer* timer, bool histogramReady = false, int minPasses = 1,
                             const uint32_t* clearHash = nullptr, uint32_t* cellStart = nullptr,
                             uint32_t* cellEnd = nullptr, const SortGather* gather = nullptr);
+// Sort of n <= RS_TILE pairs in one single-block launch that also does the
+// cell-table work of a system step: resets the cells of clearHash, gathers
+// the state (gather), and writes cellStart/cellEnd of the sorted keys.
+// Bit-identical to radix_sort_pairs (with the same clearHash and gather)
+// followed by launch_cell_bounds.  The digit-0 histogram of keysIn is not
+// read; H[0] is zeroed for the next sort.
+hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t* keysIn,
+                             const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
+                             uint32_t n, int passes, const uint32_t* clearHash,
+                             uint32_t* cellStart, uint32_t* cellEnd, uint32_t numCells,
+                             const SortGather& gather);
 // The cell-table half of reorderDataAndFindCellStart: cellStart/cellEnd of
 // the sorted hashes (the tables already hold EMPTY for every other cell).
 hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,

@@ -114,13 +114,23 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   ga.vel = sys->vel;
   ga.sortedPos = sys->sortedPos;
   ga.sortedVel = sys->sortedVel;
-  e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index, n,
-                       bits, timer, /*histogramReady=*/true, 1, nullptr, nullptr, nullptr, &ga);
-  if (e != hipSuccess) return hip_error(e, "sort");
-  if (timer) timer->begin(st, "cell_bounds");
-  e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
-  if (timer) timer->end(st);
-  if (e != hipSuccess) return hip_error(e, "cell bounds");
+  if (n <= (uint32_t)RS_TILE) {
+    if (timer) timer->begin(st, "sort_small");
+    e = launch_sort_small(st, ctx->sort, sys->keys, nullptr, sys->hash, sys->index, n,
+                          radix_passes(bits, false), nullptr, sys->cellStart, sys->cellEnd,
+                          P.numCells, ga);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "sort");
+  } else {
+    e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, nullptr, sys->hash, sys->index,
+                         n, bits, timer, /*histogramReady=*/true, 1, nullptr, nullptr, nullptr,
+                         &ga);
+    if (e != hipSuccess) return hip_error(e, "sort");
+    if (timer) timer->begin(st, "cell_bounds");
+    e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "cell bounds");
+  }
   if (timer) timer->begin(st, "collide");
   e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                      sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr,
@@ -200,17 +210,27 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     ga.idOfSlot = sys->idOfSlot;
     ga.indexOut = sys->index;
   }
-  hipError_t e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys,
-                                  fromSlots ? sys->slotOf : nullptr, sys->hash,
-                                  fromSlots ? sys->slotOf : sys->index, n,
-                                  key_bits_for_cells(P.numCells), timer, true, passes,
-                                  clearInSort ? sys->hash : nullptr, sys->cellStart, sys->cellEnd,
-                                  &ga);
-  if (e != hipSuccess) return hip_error(e, "sort");
-  if (timer) timer->begin(st, "cell_bounds");
-  e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
-  if (timer) timer->end(st);
-  if (e != hipSuccess) return hip_error(e, "cell bounds");
+  const uint32_t* valsIn = fromSlots ? sys->slotOf : nullptr;
+  uint32_t* valsOut = fromSlots ? sys->slotOf : sys->index;
+  const uint32_t* clearHash = clearInSort ? sys->hash : nullptr;
+  hipError_t e;
+  if (n <= (uint32_t)RS_TILE) {
+    // One block sorts, gathers and writes the cell tables.
+    if (timer) timer->begin(st, "sort_small");
+    e = launch_sort_small(st, ctx->sort, sys->keys, valsIn, sys->hash, valsOut, n, passes,
+                          clearHash, sys->cellStart, sys->cellEnd, P.numCells, ga);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "sort");
+  } else {
+    e = radix_sort_pairs(st, ctx->sort, ctx->devFlags, sys->keys, valsIn, sys->hash, valsOut, n,
+                         key_bits_for_cells(P.numCells), timer, true, passes, clearHash,
+                         sys->cellStart, sys->cellEnd, &ga);
+    if (e != hipSuccess) return hip_error(e, "sort");
+    if (timer) timer->begin(st, "cell_bounds");
+    e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "cell bounds");
+  }
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,

@@ -341,7 +341,144 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   if (countNext) histogram_flush(s_nextHist, gHist, pass + 1);
 }
 
+// The whole sort of n <= RS_TILE keys in one block, for small systems whose
+// step would otherwise be four launches of one block each: every pass ranks
+// the keys in LDS exactly as k_rs_scatter ranks a tile (wave ballots,
+// wave-private digit counts, block scan), which in a single tile is the
+// global rank, so no histogram, look-back or further launch is needed.
+// The block also does the other work of those launches: it resets the
+// cells of the previous sorted hashes (clearHash) and zeroes H[0] for the
+// next sort (as pass 1 does), gathers the state (SortGather) and writes the
+// cell bounds of the sorted keys (k_cell_bounds).  Outputs are bit-identical
+// to radix_sort_pairs + launch_cell_bounds.
+__global__ __launch_bounds__(RS_THREADS) void k_sort_small(
+    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int passes,
+    const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
+    uint32_t* __restrict__ cellEnd, uint32_t numCells, SortGather ga, uint32_t* gHist) {
+  __shared__ uint32_t s_keys[RS_TILE];
+  __shared__ uint32_t s_vals[RS_TILE];
+  __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
+  __shared__ uint32_t s_tileStart[RS_RADIX];
+  __shared__ uint32_t s_scratch[RS_RADIX / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t waveBase = wave * kWaveKeys;
+  {
+    uint32_t* h0 = hist_buffer(gHist, 0);
+    for (int i = threadIdx.x; i < RS_HIST_COPIES * RS_RADIX; i += RS_THREADS) h0[i] = 0u;
+  }
+  if (clearHash != nullptr) {
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = i * RS_THREADS + threadIdx.x;
+      if (idx < n) {
+        const uint32_t h = clearHash[idx];
+        if (idx == 0 || clearHash[idx - 1] != h) {
+          cellStart[h] = CELL_EMPTY;
+          cellEnd[h] = CELL_EMPTY;
+        }
+      }
+    }
+  }
+  uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const uint32_t idx = waveBase + i * 64u + lane;
+    const bool valid = idx < n;
+    key[i] = valid ? keysIn[idx] : 0xffffffffu;
+    val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
+  }
+  for (int pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = (uint32_t)pass * RS_RADIX_BITS;
+    for (int i = threadIdx.x; i < kWaves * RS_RADIX; i += RS_THREADS) (&s_waveHist[0][0])[i] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = waveBase + i * 64u + lane;
+      const bool valid = idx < n;
+      const uint64_t active = __ballot(valid);
+      const uint32_t d = (key[i] >> shift) & (RS_RADIX - 1);
+      const uint64_t peers = match_digit(d, active);
+      const uint32_t lower = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+      const uint32_t before = s_waveHist[wave][d];
+      rank[i] = before + lower;
+      if (valid && lower == 0u) s_waveHist[wave][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    const uint32_t d = threadIdx.x;
+    const bool digitThread = d < (uint32_t)RS_RADIX;
+    uint32_t count = 0;
+    if (digitThread) {
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_waveHist[w][d];
+        s_waveHist[w][d] = count;
+        count += c;
+      }
+    }
+    const uint32_t start = scan256_exclusive(digitThread ? count : 0u, s_scratch);
+    if (digitThread) s_tileStart[d] = start;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t idx = waveBase + i * 64u + lane;
+      if (idx < n) {
+        const uint32_t dg = (key[i] >> shift) & (RS_RADIX - 1);
+        const uint32_t lp = s_tileStart[dg] + s_waveHist[wave][dg] + rank[i];
+        s_keys[lp] = key[i];
+        s_vals[lp] = val[i];
+      }
+    }
+    __syncthreads();
+    if (pass + 1 < passes) {
+#pragma unroll
+      for (int i = 0; i < RS_ITEMS; ++i) {
+        const uint32_t idx = waveBase + i * 64u + lane;
+        key[i] = idx < n ? s_keys[idx] : 0xffffffffu;
+        val[i] = idx < n ? s_vals[idx] : 0u;
+      }
+      __syncthreads();
+    }
+  }
+  // Sorted keys and values are in s_keys/s_vals[0, n); the clears above
+  // completed before the first barrier, so the bounds below overwrite them.
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const uint32_t g = i * RS_THREADS + threadIdx.x;
+    if (g < n) {
+      const uint32_t k = s_keys[g], v = s_vals[g];
+      keysOut[g] = k;
+      valsOut[g] = v;
+      if (ga.pos && v < n) {
+        ga.sortedPos[g] = ga.pos[v];
+        const float4 vv = ga.vel[v];
+        ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
+        if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
+      }
+      const uint32_t kp = g > 0 ? s_keys[g - 1] : 0u;
+      if (g == 0 || k != kp) {
+        if (k < numCells) cellStart[k] = g;
+        if (g > 0 && kp < numCells) cellEnd[kp] = g;
+      }
+      if (g == n - 1 && k < numCells) cellEnd[k] = g + 1;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t* keysIn,
+                             const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
+                             uint32_t n, int passes, const uint32_t* clearHash,
+                             uint32_t* cellStart, uint32_t* cellEnd, uint32_t numCells,
+                             const SortGather& gather) {
+  if (n == 0) return hipSuccess;
+  if (n > (uint32_t)RS_TILE || passes < 1 || passes > RS_MAX_PASSES) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(RS_THREADS), 0, stream, keysIn, valsIn, keysOut,
+                     valsOut, n, passes, clearHash, cellStart, cellEnd, numCells, gather,
+                     s.histogram);
+  return hipGetLastError();
+}
 
 int key_bits_for_cells(uint32_t numCells) {
   if (numCells <= 1u) return 0;
