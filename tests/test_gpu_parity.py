@@ -260,6 +260,28 @@ def test_run_graph_equals_repeated_update(gpu_ctx):
     b.close()
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4095, 4096, 4097])
+def test_small_systems_run_and_update_match_oracle(gpu_ctx, n):
+    # up to one sort tile (4096) a step sorts in one block that also writes
+    # the cell tables; 4097 takes the multi-block sort
+    grid, r = 64, 1.0 / 64.0
+    print(f"[small] n={n}", flush=True)
+    for mode in ("run", "update"):
+        ps, P, pos, vel = _system(n, grid, r, seed=5)
+        st = orc.State(pos, vel, P.numCells)
+        OP = orc.params_from(P)
+        steps = 12 if mode == "run" else 3
+        if mode == "run":
+            ps.run(0.5, steps)
+        else:
+            for _ in range(steps):
+                ps.update(0.5)
+        for _ in range(steps):
+            st.update(0.5, OP)
+        _compare_system(ps, st)
+        ps.close()
+
+
 def test_full_size_steps_match_oracle(gpu_ctx):
     # 1,048,576 spheres r=1/128 on a 256^3 grid: two whole updates, bit-exact
     n, grid, r = 1 << 20, 256, 1.0 / 128.0
