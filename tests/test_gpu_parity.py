@@ -282,6 +282,28 @@ def test_small_systems_run_and_update_match_oracle(gpu_ctx, n):
         ps.close()
 
 
+@pytest.mark.parametrize("n", [4096, 20_000])
+def test_attraction_matches_oracle(gpu_ctx, n):
+    # a non-zero attraction term (skipped exactly when zero) through the
+    # single-block and multi-block sorts, single updates and fused runs
+    grid, r = 64, 1.0 / 64.0
+    print(f"[attraction] n={n}", flush=True)
+    for mode in ("run", "update"):
+        ps, P, pos, vel = _system(n, grid, r, seed=9, attraction=0.02)
+        st = orc.State(pos, vel, P.numCells)
+        OP = orc.params_from(P)
+        steps = 10 if mode == "run" else 3
+        if mode == "run":
+            ps.run(0.5, steps)
+        else:
+            for _ in range(steps):
+                ps.update(0.5)
+        for _ in range(steps):
+            st.update(0.5, OP)
+        _compare_system(ps, st)
+        ps.close()
+
+
 def test_full_size_steps_match_oracle(gpu_ctx):
     # 1,048,576 spheres r=1/128 on a 256^3 grid: two whole updates, bit-exact
     n, grid, r = 1 << 20, 256, 1.0 / 128.0
