@@ -56,7 +56,10 @@ struct fc_body_set {
   uint32_t* counters = nullptr;   // device, 3
   uint16_t* netOf = nullptr;      // device, capacity: bucket of each sphere
   uint32_t* order = nullptr;      // device, capacity: spheres in bucket order
-  uint32_t* bins = nullptr;       // device, 2 * (maxNet + 3): counts, cursors
+  uint32_t* band = nullptr;       // device, capacity: candidates of the cull
+  uint32_t* bins = nullptr;       // device: spheres in a box, then per cull wave its candidates
+  uint32_t* blockBins = nullptr;  // device: per bucketing block and bucket, count then start
+  size_t blockBinsCap = 0;
   size_t capacity = 0;
 };
 
@@ -639,112 +642,133 @@ struct LatticeArg {
 
 // Spheres are first grouped by the network of their body (one bucket per
 // network plus one for spheres outside every box) so that each 16-sphere
-// MFMA tile of the cull sees one network: count, scan, scatter.  The order
-// inside a bucket is arbitrary; every sphere's result depends only on its
-// own column of the tile.
-// Adds `valid` lanes' values v to an LDS histogram with one atomic per
-// distinct value per wave; returns each lane's rank among the wave's lanes
-// of the same value plus the count already in the bin (the lane's slot).
-__device__ __forceinline__ uint32_t wave_hist_add(uint32_t* s_cnt, uint32_t v, bool valid) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint64_t todo = __ballot(valid);
-  uint32_t slot = 0;
-  while (todo) {
-    const int leader = __ffsll((unsigned long long)todo) - 1;
-    const uint32_t lv = (uint32_t)__shfl((int)v, leader);
-    const uint64_t same = __ballot(valid && v == lv);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&s_cnt[lv], (uint32_t)__popcll(same));
-    base = (uint32_t)__shfl((int)base, leader);
-    if (valid && v == lv) slot = base + (uint32_t)__popcll(same & below);
-    todo &= ~same;
-  }
-  return slot;
-}
+// MFMA tile of the cull sees one network: count, scan, scatter, with plain
+// LDS atomics.  The order inside a bucket is arbitrary (it follows the
+// atomics); every sphere's result depends only on its own column of the
+// tile, so the results do not.
+// Bucketing blocks: 256 threads, 16 spheres per thread (few blocks, so the
+// per-block bucket counts the scan walks stay few).
+constexpr uint32_t kBT = 256;
+constexpr uint32_t kBucketChunk = 16 * kBT;
 
-// Spheres per block of the bucketing kernels (16 per thread): few blocks,
-// so few same-address global atomics per bucket.
-constexpr uint32_t kBucketChunk = 16 * kTB;
+// Also clears candidate[i] and contact[i] of every sphere (in index order;
+// the band query sets them for the cull's candidates).  The spheres are
+// taken kBucketBatch at a time with their loads issued together.
+constexpr uint32_t kBucketBatch = 16;
 
-__global__ __launch_bounds__(kTB) void k_bucket_count(
+__global__ __launch_bounds__(kBT) void k_bucket_count(
     const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
     uint32_t B, const float4* __restrict__ pos, uint32_t n, uint32_t nets,
-    uint16_t* __restrict__ netOf, uint32_t* __restrict__ counts) {
+    uint16_t* __restrict__ netOf, uint32_t* __restrict__ blockCounts,
+    uint8_t* __restrict__ candidate, uint8_t* __restrict__ contact) {
   extern __shared__ uint32_t s_cnt[];
-  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) s_cnt[k] = 0u;
+  for (uint32_t k = threadIdx.x; k <= nets; k += kBT) s_cnt[k] = 0u;
   __syncthreads();
-  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {
-    const uint32_t i = blockIdx.x * kBucketChunk + u * kTB + threadIdx.x;
-    uint32_t net = nets;  // nets = outside every box
-    if (i < n) {
-      v3f q;
-      const int b = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B,
-                            xyz(pos[i]), q);
-      if (b >= 0 && in_box(bodies[b].map, q)) net = (uint32_t)bodies[b].net;
-      netOf[i] = (uint16_t)net;
+  for (uint32_t u0 = 0; u0 < kBucketChunk / kBT; u0 += kBucketBatch) {
+    float4 p[kBucketBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBucketBatch; ++u) {
+      const uint32_t i = blockIdx.x * kBucketChunk + (u0 + u) * kBT + threadIdx.x;
+      p[u] = i < n ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    wave_hist_add(s_cnt, net, i < n);
+    uint32_t net[kBucketBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBucketBatch; ++u) {
+      const uint32_t i = blockIdx.x * kBucketChunk + (u0 + u) * kBT + threadIdx.x;
+      net[u] = nets;  // nets = outside every box
+      if (i < n) {
+        v3f q;
+        const int b = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B,
+                              xyz(p[u]), q);
+        if (b >= 0 && in_box(bodies[b].map, q)) net[u] = (uint32_t)bodies[b].net;
+        netOf[i] = (uint16_t)net[u];
+        candidate[i] = 0;
+        contact[i] = 0;
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kBucketBatch; ++u) {
+      const uint32_t i = blockIdx.x * kBucketChunk + (u0 + u) * kBT + threadIdx.x;
+      if (i < n) atomicAdd(&s_cnt[net[u]], 1u);
+    }
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k <= nets; k += kTB)
-    if (s_cnt[k]) atomicAdd(&counts[k], s_cnt[k]);
+  for (uint32_t k = threadIdx.x; k <= nets; k += kBT)
+    blockCounts[(size_t)k * gridDim.x + blockIdx.x] = s_cnt[k];
 }
 
-// Exclusive scan of the nets + 1 bucket counts (one block): cursor[k] =
-// start of bucket k; cursor[nets + 1] = n.
-__global__ __launch_bounds__(kTB) void k_bucket_scan(const uint32_t* __restrict__ counts,
-                                                     uint32_t nets, uint32_t* __restrict__ cursor) {
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (uint32_t k = 0; k <= nets; ++k) {
-      cursor[k] = acc;
-      acc += counts[k];
-    }
-    cursor[nets + 1] = acc;
+// Exclusive scan, in one block, of the per-block bucket counts laid out
+// bucket by bucket (blockCounts[k * blocks + b]): afterwards entry (k, b) is
+// where block b's spheres of bucket k start, so the buckets are contiguous
+// and ordered by block inside.  inBox = the start of the last bucket (the
+// spheres outside every box).  No global atomics: a few hundred blocks
+// adding to the same few dozen bucket counters serialise on them.
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_bucket_scan(uint32_t* __restrict__ blockCounts,
+                                                              uint32_t entries, uint32_t lastBucket,
+                                                              uint32_t* __restrict__ inBox) {
+  __shared__ uint32_t s_part[kScanThreads];
+  const uint32_t per = (entries + kScanThreads - 1) / kScanThreads;
+  const uint32_t lo = threadIdx.x * per, hi = lo + per < entries ? lo + per : entries;
+  uint32_t sum = 0;
+  for (uint32_t e = lo; e < hi; ++e) sum += blockCounts[e];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // inclusive Hillis-Steele
+    const uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s_part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t acc = threadIdx.x == 0 ? 0u : s_part[threadIdx.x - 1];
+  for (uint32_t e = lo; e < hi; ++e) {
+    const uint32_t c = blockCounts[e];
+    blockCounts[e] = acc;
+    if (e == lastBucket) *inBox = acc;
+    acc += c;
   }
 }
 
-__global__ __launch_bounds__(kTB) void k_bucket_scatter(const uint16_t* __restrict__ netOf,
+__global__ __launch_bounds__(kBT) void k_bucket_scatter(const uint16_t* __restrict__ netOf,
                                                         uint32_t n, uint32_t nets,
-                                                        uint32_t* __restrict__ cursor,
+                                                        const uint32_t* __restrict__ blockStart,
                                                         uint32_t* __restrict__ order) {
   extern __shared__ uint32_t s_cnt[];
   uint32_t* s_base = s_cnt + nets + 1;
-  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) s_cnt[k] = 0u;
+  for (uint32_t k = threadIdx.x; k <= nets; k += kBT) {
+    s_cnt[k] = 0u;
+    s_base[k] = blockStart[(size_t)k * gridDim.x + blockIdx.x];
+  }
   __syncthreads();
   const uint32_t base = blockIdx.x * kBucketChunk;
-  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {  // this block's counts
-    const uint32_t i = base + u * kTB + threadIdx.x;
-    wave_hist_add(s_cnt, i < n ? (uint32_t)netOf[i] : 0u, i < n);
-  }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k <= nets; k += kTB) {
-    s_base[k] = s_cnt[k] ? atomicAdd(&cursor[k], s_cnt[k]) : 0u;
-    s_cnt[k] = 0u;
-  }
-  __syncthreads();
-  for (uint32_t u = 0; u < kBucketChunk / kTB; ++u) {  // slots
-    const uint32_t i = base + u * kTB + threadIdx.x;
-    const uint32_t net = i < n ? (uint32_t)netOf[i] : 0u;
-    const uint32_t slot = wave_hist_add(s_cnt, net, i < n);
-    if (i < n) order[s_base[net] + slot] = i;
+#pragma unroll
+  for (uint32_t u = 0; u < kBucketChunk / kBT; ++u) {
+    const uint32_t i = base + u * kBT + threadIdx.x;
+    if (i < n) {
+      const uint32_t net = (uint32_t)netOf[i];
+      order[s_base[net] + atomicAdd(&s_cnt[net], 1u)] = i;
+    }
   }
 }
 
 // Surrogate cull over the bucketed spheres: the BF16 network predicts each
-// in-box sphere's distance; candidate[i] = 1 for those within radius +
-// margin[body] (0 for every other sphere).
+// in-box sphere's distance, and those within radius + margin[body] are
+// listed for the exact query: wave w lists its candidates from band[16 t0]
+// on (t0 = its first tile) and their number in waveCount[w].
 __global__ __launch_bounds__(kTB) void k_boundary_cull(
     const uint16_t* __restrict__ packed, const float* __restrict__ params,
     const float* __restrict__ freq, const float* __restrict__ frame,
     const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
     uint32_t B, const float* __restrict__ margins, const float4* __restrict__ pos,
-    const uint32_t* __restrict__ order, uint32_t n, float radius,
-    uint8_t* __restrict__ candidate) {
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ inBoxCount, float radius,
+    uint32_t* __restrict__ band, uint32_t* __restrict__ waveCount) {
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const uint32_t wave = (blockIdx.x * kTB + threadIdx.x) >> 6;
   const uint32_t nWaves = (gridDim.x * kTB) >> 6;
+  // only the in-box buckets (the spheres outside every box come last and
+  // were cleared by the bucketing), shared evenly among the waves
+  const uint32_t n = *inBoxCount;
   const uint32_t tiles = (n + 15) / 16;
   // contiguous runs of tiles per wave: consecutive tiles share a network,
   // so the weight fragments are reloaded only where the network changes
@@ -752,6 +776,8 @@ __global__ __launch_bounds__(kTB) void k_boundary_cull(
   const uint32_t t0 = wave * per, t1 = t0 + per < tiles ? t0 + per : tiles;
   bf16x8 frag[kSdfFrags];
   int cur = -1;
+  // the wave's candidates are listed from band[16 t0] on (its slots' range)
+  uint32_t listed = 0;
   for (uint32_t tile = t0; tile < t1; ++tile) {
     const uint32_t slot = tile * 16 + col;
     uint32_t i = 0;
@@ -786,19 +812,61 @@ __global__ __launch_bounds__(kTB) void k_boundary_cull(
       if (mine) cand = yv * fr[3] <= radius + margins[body];
       remaining = remaining && !mine;
     }
-    if (slot < n && g == 0) candidate[i] = cand ? 1 : 0;
+    // candidates go to the wave's band list
+    const bool listMe = slot < n && g == 0 && cand;
+    const uint64_t cmask = __ballot(listMe);
+    if (listMe) band[t0 * 16u + listed + (uint32_t)__popcll(cmask & ((1ull << lane) - 1ull))] = i;
+    listed += (uint32_t)__popcll(cmask);
+  }
+  if (lane == 0) waveCount[wave] = listed;
+}
+
+// The exact query of the cull's candidates: block w walks the list of cull
+// wave w (waveCount[w] entries from band[16 t0(w)] on, t0 as in the cull);
+// contact[i] = 1 when the map distance is < radius.  counters (optional):
+// [0] = spheres in a box, [1] += candidates, [2] += contacts.
+constexpr uint32_t kBandThreads = 128;
+
+__global__ __launch_bounds__(kBandThreads) void k_boundary_band(
+    const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
+    uint32_t B, const float4* __restrict__ pos, const uint32_t* __restrict__ band,
+    const uint32_t* __restrict__ waveCount, uint32_t cullWaves,
+    const uint32_t* __restrict__ inBoxCount, float radius, uint8_t* __restrict__ candidate,
+    uint8_t* __restrict__ contact, uint32_t* __restrict__ counters) {
+  const uint32_t w = blockIdx.x;
+  const uint32_t tiles = (*inBoxCount + 15) / 16;
+  const uint32_t per = (tiles + cullWaves - 1) / cullWaves;
+  const uint32_t first = w * per * 16u;
+  const uint32_t m = (w * per < tiles) ? waveCount[w] : 0u;
+  if (counters && threadIdx.x == 0) {
+    if (w == 0) counters[0] = *inBoxCount;
+    if (m) atomicAdd(&counters[1], m);
+  }
+  for (uint32_t e = threadIdx.x; e < m; e += kBandThreads) {
+    const uint32_t i = band[first + e];
+    v3f q;
+    const int b = body_of(bodies, cellBody, L.lx, L.ly, L.lz, L.ox, L.oy, L.oz, L.cell, B,
+                          xyz(pos[i]), q);
+    uint8_t c = 0;
+    if (b >= 0) {  // always: the cull found the same body
+      float d = 0.0f;
+      v3f cp;
+      distmap_query(bodies[b].map, q, d, cp, nullptr);
+      c = d < radius ? 1 : 0;
+    }
+    candidate[i] = 1;
+    contact[i] = c;
+    if (counters && c) atomicAdd(&counters[2], 1u);
   }
 }
 
 // Exact contact test, one lane per sphere: contact[i] = 1 when the sphere's
 // body's map distance is < radius (checkDistMap's test, boundaries.cuh:51).
-// With a candidate mask only the flagged spheres are queried.  counters
-// (optional): spheres in a box, exact queries, contacts.
+// counters (optional): spheres in a box, exact queries, contacts.
 __global__ __launch_bounds__(kTB) void k_boundary_exact(
     const BodyDev* __restrict__ bodies, const int32_t* __restrict__ cellBody, LatticeArg L,
     uint32_t B, const float4* __restrict__ pos, uint32_t n, float radius,
-    const uint8_t* __restrict__ candidate, uint8_t* __restrict__ contact,
-    uint32_t* __restrict__ counters) {
+    uint8_t* __restrict__ contact, uint32_t* __restrict__ counters) {
   const uint32_t i = blockIdx.x * kTB + threadIdx.x;
   if (i >= n) return;
   v3f q;
@@ -806,7 +874,7 @@ __global__ __launch_bounds__(kTB) void k_boundary_exact(
                         xyz(pos[i]), q);
   const bool inside = b >= 0 && in_box(bodies[b].map, q);
   uint8_t c = 0;
-  const bool query = inside && (candidate == nullptr || candidate[i] != 0);
+  const bool query = inside;
   if (query) {
     float d = 0.0f;
     v3f cp;
@@ -867,13 +935,16 @@ void free_sdf(fc_sdf* m) {
 }
 
 constexpr uint32_t kTilesPerBlock = 8;
+constexpr uint32_t kMaxCullWaves = 2048;
 
-// Blocks (of 4 waves) of the MFMA kernels for n queries: up to 3 resident
-// waves per SIMD of the 256 CUs, each wave taking at least 16 tiles.
+// Blocks (of 4 waves) of the MFMA kernels for n queries: at most the waves
+// that are resident at once (the kernels hold the 20 weight fragments in
+// registers, 170 VGPRs: two waves per SIMD of the 256 CUs), each wave
+// taking at least 16 tiles, so no wave waits for a second round.
 uint32_t mfma_blocks(uint32_t n) {
   const uint32_t tiles = (n + 15) / 16;
   uint32_t waves = (tiles + 15) / 16;
-  if (waves > 3072u) waves = 3072u;
+  if (waves > kMaxCullWaves) waves = kMaxCullWaves;
   const uint32_t blocks = (waves + 3) / 4;
   return blocks == 0 ? 1u : blocks;
 }
@@ -1147,7 +1218,9 @@ fc_status fc_body_set_destroy(fc_body_set* s) {
   if (!s) return FC_SUCCESS;
   hipFree(s->netOf);
   hipFree(s->order);
+  hipFree(s->band);
   hipFree(s->bins);
+  hipFree(s->blockBins);
   hipFree(s->bodies);
   hipFree(s->cellBody);
   hipFree(s->margins);
@@ -1184,45 +1257,64 @@ fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrog
       FC_HIP_TRY(hipStreamSynchronize(st));
       hipFree(set->netOf);
       hipFree(set->order);
+      hipFree(set->band);
       set->netOf = nullptr;
       set->order = nullptr;
+      set->band = nullptr;
       set->capacity = 0;
       FC_HIP_TRY(hipMalloc(&set->netOf, (size_t)n * sizeof(uint16_t)));
       FC_HIP_TRY(hipMalloc(&set->order, (size_t)n * sizeof(uint32_t)));
+      FC_HIP_TRY(hipMalloc(&set->band, (size_t)n * sizeof(uint32_t)));
       set->capacity = n;
     }
-    if (!set->bins) FC_HIP_TRY(hipMalloc(&set->bins, 2 * 4098 * sizeof(uint32_t)));
-    uint32_t* cnt = set->bins;
-    uint32_t* cursor = set->bins + 4098;
+    if (!set->bins) FC_HIP_TRY(hipMalloc(&set->bins, (1 + kMaxCullWaves) * sizeof(uint32_t)));
+    uint32_t* inBox = set->bins;
+    uint32_t* waveCount = set->bins + 1;
+    const unsigned blocks = (n + kBucketChunk - 1) / kBucketChunk;
+    const size_t entries = (size_t)(nets + 1) * blocks;
+    if (set->blockBinsCap < entries) {
+      FC_HIP_TRY(hipStreamSynchronize(st));
+      hipFree(set->blockBins);
+      set->blockBins = nullptr;
+      set->blockBinsCap = 0;
+      FC_HIP_TRY(hipMalloc(&set->blockBins, entries * sizeof(uint32_t)));
+      set->blockBinsCap = entries;
+    }
     if (counts) FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
     if (timer) timer->begin(st, "boundary_bucket");
-    FC_HIP_TRY(launch_fill_u32(st, cnt, 0u, nets + 1));
-    const unsigned blocks = (n + kBucketChunk - 1) / kBucketChunk;
-    hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(kTB), (nets + 1) * sizeof(uint32_t), st,
+    hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(kBT), (nets + 1) * sizeof(uint32_t), st,
                        set->bodies, set->cellBody, L, set->B, (const float4*)pos, n, nets,
-                       set->netOf, cnt);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kTB), 0, st, cnt, nets, cursor);
-    hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks), dim3(kTB), 2 * (nets + 1) * sizeof(uint32_t),
-                       st, set->netOf, n, nets, cursor, set->order);
+                       set->netOf, set->blockBins, candidate, contact);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kScanThreads), 0, st, set->blockBins,
+                       (uint32_t)entries, (uint32_t)((size_t)nets * blocks), inBox);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks), dim3(kBT), 2 * (nets + 1) * sizeof(uint32_t),
+                       st, set->netOf, n, nets, set->blockBins, set->order);
     if (timer) timer->end(st);
     FC_HIP_TRY(hipGetLastError());
     const uint32_t cblocks = mfma_blocks(n);
     if (timer) timer->begin(st, "boundary_cull");
     hipLaunchKernelGGL(k_boundary_cull, dim3(cblocks), dim3(kTB), 0, st, surrogate->packed,
                        surrogate->params, surrogate->freq, surrogate->frame, set->bodies,
-                       set->cellBody, L, set->B, set->margins, (const float4*)pos, set->order, n,
-                       radius, candidate);
+                       set->cellBody, L, set->B, set->margins, (const float4*)pos, set->order,
+                       inBox, radius, set->band, waveCount);
     if (timer) timer->end(st);
     FC_HIP_TRY(hipGetLastError());
-  } else if (counts) {
-    FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
+    if (timer) timer->begin(st, "boundary_exact_band");
+    hipLaunchKernelGGL(k_boundary_band, dim3(cblocks * 4), dim3(kBandThreads), 0, st,
+                       set->bodies, set->cellBody, L, set->B, (const float4*)pos, set->band,
+                       waveCount, cblocks * 4, inBox, radius, candidate, contact,
+                       counts ? set->counters : nullptr);
+    if (timer) timer->end(st);
+    FC_HIP_TRY(hipGetLastError());
+  } else {
+    if (counts) FC_HIP_TRY(hipMemsetAsync(set->counters, 0, 3 * sizeof(uint32_t), st));
+    if (timer) timer->begin(st, "boundary_exact");
+    hipLaunchKernelGGL(k_boundary_exact, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, st,
+                       set->bodies, set->cellBody, L, set->B, (const float4*)pos, n, radius,
+                       contact, counts ? set->counters : nullptr);
+    if (timer) timer->end(st);
+    FC_HIP_TRY(hipGetLastError());
   }
-  if (timer) timer->begin(st, surrogate ? "boundary_exact_band" : "boundary_exact");
-  hipLaunchKernelGGL(k_boundary_exact, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, st, set->bodies,
-                     set->cellBody, L, set->B, (const float4*)pos, n, radius,
-                     surrogate ? candidate : nullptr, contact, counts ? set->counters : nullptr);
-  if (timer) timer->end(st);
-  FC_HIP_TRY(hipGetLastError());
   if (counts) {
     FC_HIP_TRY(hipMemcpyAsync(counts, set->counters, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     FC_HIP_TRY(hipStreamSynchronize(st));
