@@ -77,19 +77,20 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
     (the multi-step run, which keeps the particle state in sorted-slot order
     between fused steps) or "update" (single updates)."""
     run = path == "run"
-    if name == "collide":  # own pos/vel 32 B, index 4 B, newVel 16 B (the neighbours'
-        # hashes ride in sortedVel.w); + the positions scattered back to id order by a
-        # run's last step
-        return 52 * n + (16 * n if run else 0) + 8 * c_touch
-    if name == "collide_integrate":  # own pos/vel 32, index 4; pos/vel 32, key 4, slot 4,
-        return 80 * n + 8 * c_touch  # id of slot 4 (newVel store/reload fused away)
+    if name == "collide":  # own pos/vel 32 B, fixed-point coordinates 8 B, index 4 B,
+        # newVel 16 B (the neighbours' hashes ride in sortedVel.w); + the positions
+        # scattered back to id order by a run's last step
+        return 60 * n + (16 * n if run else 0) + 8 * c_touch
+    if name == "collide_integrate":  # own pos/vel 32, coordinates 8, index 4; pos/vel 32,
+        return 88 * n + 8 * c_touch  # key 4, slot 4, id of slot 4 (newVel fused away)
     if name == "cell_bounds":  # sorted hashes 4 B, cellStart/cellEnd per occupied cell
         return 4 * n + 8 * c_occ
     if name.startswith("sort_pass"):  # keys+values read and written once; the last pass
-        # also gathers pos/vel into sorted order (32 B read, 32 B written) and, in a run,
-        # writes the ids through the slot map (4 B read, 4 B written)
+        # also gathers pos/vel into sorted order (32 B read, 32 B written), writes the
+        # fixed-point coordinates (8 B) and, in a run, the ids through the slot map (4 B
+        # read, 4 B written)
         last = int(name[len("sort_pass"):]) == SORT_PASSES - 1
-        return 16 * n + ((64 * n + (8 * n if run else 0)) if last else 0)
+        return 16 * n + ((72 * n + (8 * n if run else 0)) if last else 0)
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
         return 72 * n
     return None

@@ -213,6 +213,19 @@ __device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteO
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byteOff);
 }
 
+// Squared distance of two packed fixed-point coordinate triples
+// (quantize_pos): the per-axis differences in 16-bit lanes, then two packed
+// dot products.  Exact whenever every difference fits int16 (the sum of
+// three squares of at most 2^15 stays below 2^32); a larger difference
+// wraps and may read short, which only keeps a far pair as a candidate.
+__device__ __forceinline__ uint32_t quant_d2(uint2 a, uint2 b) {
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  const s16x2 dxy = __builtin_bit_cast(s16x2, a.x) - __builtin_bit_cast(s16x2, b.x);
+  const s16x2 dz = __builtin_bit_cast(s16x2, a.y) - __builtin_bit_cast(s16x2, b.y);
+  return (uint32_t)__builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, 0, false),
+                                          false);
+}
+
 // Row-range variant of collide_one_v3 for grids with every dimension >= 4,
 // numCells <= 2^30 and n < 2^28, on the arrays of a sort that gathered the
 // state (SortGather): sortedVel[j].w holds the bits of sphere j's sorted
@@ -227,11 +240,15 @@ __device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteO
 // direct loop.)  Cells out of contact reach are cut from the ranges first.
 //
 // Phase A walks the three rows of each z-plane as one flattened sequence
-// and keeps the byte offsets of candidates that pass a pre-filter in a
-// per-lane list in LDS.  The pre-filter evaluates d2 with fused
-// multiply-adds and compares against contactD2 raised by 2^-18 relative,
-// which keeps every candidate with (reference-rounded) d2 < contactD2; the
-// extra near-misses get the exact contact test in phase B and add +0.
+// and keeps the sorted indices of candidates that pass a pre-filter in a
+// per-lane list in LDS.  The pre-filter reads 8 bytes per candidate, the
+// 16-bit fixed-point coordinates sortedQ, and keeps every candidate whose
+// exact integer squared distance to this sphere's is below qThreshold, a
+// bound that every pair in contact (by the reference's rounded distance)
+// stays under: rounding moves a fixed-point distance by less than the
+// bound's margin and clamping into the frame only shortens it.  The extra
+// near-misses get the exact contact test in phase B and add +0; so does a
+// pair with a NaN sphere, whatever its coordinates read.
 // Phase B walks the list and detects a change of cell by a change of the
 // neighbour's sorted hash (all 27 cells are distinct when every grid
 // dimension is >= 3), adding each cell's partial to the total in cell
@@ -243,6 +260,8 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
                                                 float contactD2, uint32_t k, v3f pos, v3f vel,
                                                 float rA, const float4* __restrict__ sortedPos,
                                                 const float4* __restrict__ sortedVel,
+                                                const uint2* __restrict__ sortedQ,
+                                                uint32_t qThreshold,
                                                 const uint32_t* __restrict__ cellStart,
                                                 const uint32_t* __restrict__ cellEnd, uint32_t n,
                                                 uint32_t (*s_list)[BS]) {
@@ -322,11 +341,11 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
   // The three rows of a z-plane are walked as one flattened sequence of
   // slots, kRowsBatch per iteration with their loads in flight together.
   // Slot t of the plane is sphere t + q(t), q stepping at the row
-  // boundaries; the list keeps the candidates' byte offsets.  A slot
+  // boundaries; the list keeps the candidates' sorted indices.  A slot
   // past the plane's end reads at most kRowsBatch - 1 entries past the last
   // row's range (inside the array's padding) and its test is masked off.
   // Only the centre plane holds the sphere itself.
-  const uint32_t k16 = k << 4;
+  const uint2 me = sortedQ[k];
   uint32_t cnt = 0;
 #pragma unroll
   for (int pl = 0; pl < 3; ++pl) {
@@ -335,21 +354,20 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
                    q2 = lo[3 * pl + 2] - b2;  // modulo 2^32
 #pragma unroll 1
     for (uint32_t t = 0; t < tot; t += kRowsBatch) {
-      uint32_t off[kRowsBatch];
-      float4 qq[kRowsBatch];
+      uint32_t idx[kRowsBatch];
+      uint2 qq[kRowsBatch];
 #pragma unroll
       for (int u = 0; u < kRowsBatch; ++u) {
         const uint32_t tu = t + (uint32_t)u;
-        off[u] = (tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2))) << 4;
-        qq[u] = load_off(sortedPos, off[u]);
+        idx[u] = tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2));
+        qq[u] = load_off(sortedQ, idx[u] << 3);
       }
 #pragma unroll
       for (int u = 0; u < kRowsBatch; ++u) {
-        const float ex = qq[u].x - pos.x, ey = qq[u].y - pos.y, ez = qq[u].z - pos.z;
-        const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = off[u];
-        const bool in = (u == 0 || t + (uint32_t)u < tot) && (pl != 1 || off[u] != k16);
-        cnt += (d2 < thrA && in) ? 1u : 0u;
+        const uint32_t d2 = quant_d2(qq[u], me);
+        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = idx[u];
+        const bool in = (u == 0 || t + (uint32_t)u < tot) && (pl != 1 || idx[u] != k);
+        cnt += (d2 < qThreshold && in) ? 1u : 0u;
       }
     }
   }
@@ -367,8 +385,9 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
 #pragma unroll
       for (int u = 0; u < kPairBatch; ++u) {
         const uint32_t iu = i + (uint32_t)u;
-        uint32_t off = s_list[iu < (uint32_t)kListCap ? iu : (uint32_t)kListCap][threadIdx.x];
-        off = (u == 0 || iu < cnt) ? off : k16;  // a stale entry may lie in the padding
+        uint32_t j = s_list[iu < (uint32_t)kListCap ? iu : (uint32_t)kListCap][threadIdx.x];
+        j = (u == 0 || iu < cnt) ? j : k;  // a stale entry may lie in the padding
+        const uint32_t off = j << 4;
         qp[u] = load_off(sortedPos, off);
         qv[u] = load_off(sortedVel, off);  // .w holds the sorted hash
       }

@@ -262,7 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
 // particleRadius raises DEVERR_RADIUS: the neighbour search and the contact
 // pre-filter are sized by particleRadius).
 // kRows selects the row-range neighbour walk (collide_one_rows), which
-// needs the sorted hashes.
+// needs the sorted hashes in sortedVel.w and the fixed-point coordinates
+// sortedQ with their filter bound qThreshold.
 template <bool kFuse, bool kPoly, bool kRows>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
     fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
     uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
-    v3f invCell) {
+    v3f invCell, const uint2* __restrict__ sortedQ, uint32_t qThreshold) {
   __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -288,7 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const v3f nv =
         kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, xyz(p4),
                                                 xyz(sortedVel[k]), rA, sortedPos, sortedVel,
-                                                cellStart, cellEnd, n, s_list)
+                                                sortedQ, qThreshold, cellStart, cellEnd, n,
+                                                s_list)
               : collide_one_v3<kPoly, kBlock>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
                                               sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
     const uint32_t orig = index[k];
@@ -397,11 +399,11 @@ float contact_d2_threshold(float R) {
 }
 
 // The row-range walk needs arrays gathered by the sort (the sorted hash in
-// sortedVel.w, kSortedPosPad entries of padding after sortedPos: a system's
-// own arrays), 27 distinct neighbour cells (every grid dimension >= 4 here)
-// and cell byte offsets below 2^32.
-static bool rows_walk_ok(const fc_sim_params& P, bool gathered) {
-  return gathered && P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
+// sortedVel.w, the fixed-point coordinates, kSortedPosPad entries of
+// padding: a system's own arrays, GatheredState), 27 distinct neighbour
+// cells (every grid dimension >= 4 here) and cell byte offsets below 2^32.
+static bool rows_walk_ok(const fc_sim_params& P) {
+  return P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
          P.numCells <= (1u << 30);
 }
 
@@ -422,30 +424,46 @@ static v3f inv_cell(const fc_sim_params& P) {
   return v3f{ix, iy, iz};
 }
 
+// Bound on the squared fixed-point distance (in units of 1/scale) of every
+// pair closer than R, for the row-range collide's candidate filter: each
+// coordinate lies within 0.51 units of (p - lo) * scale after the float
+// rounding of that product and rint, so a pair's difference vector is
+// within 1.02 * sqrt(3) < 1.8 units of the scaled true one; the bound adds
+// 2.5 units, which also covers the rounding of the reference's own distance
+// (~1e-7 relative).  Every pair is a candidate when the bound leaves the
+// int16 range of the coordinate differences.
+static uint32_t quant_threshold(float R, float scale) {
+  const double b = (double)R * (double)scale + 2.5;
+  if (!(b < 32000.0)) return 0xffffffffu;
+  return (uint32_t)std::ceil(b * b);
+}
+
 template <bool kFuse, bool kPoly>
-static void launch_v3(hipStream_t s, bool rows, dim3 g, dim3 b, const fc_sim_params& P,
+static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
+                      const fc_sim_params& P,
                       const BoundaryArg& B, float d2, float4* newVel, const float4* sortedPos,
                       const float4* sortedVel, const uint32_t* index, const uint32_t* cellStart,
                       const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
                       uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot) {
-  if (rows)
+  if (gs != nullptr && rows_walk_ok(P))
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       inv_cell(P));
+                       inv_cell(P), gs->sortedQ,
+                       quant_threshold(P.particleRadius + P.particleRadius, gs->frame.scale));
   else
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       inv_cell(P));
+                       inv_cell(P), nullptr, 0u);
 }
 
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut,
-                          bool gathered) {
+                          const GatheredState* gathered) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
@@ -457,13 +475,12 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
                        cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
     return hipGetLastError();
   }
-  const bool rows = rows_walk_ok(P, gathered);
   if (poly)
-    launch_v3<false, true>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
+    launch_v3<false, true>(s, gathered, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                            index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
                            nullptr, devFlags, nullptr, nullptr);
   else
-    launch_v3<false, false>(s, rows, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
+    launch_v3<false, false>(s, gathered, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                             index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
                             nullptr, devFlags, nullptr, nullptr);
   return hipGetLastError();
@@ -479,17 +496,17 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, uint32_t* idOfSlot, bool gathered) {
+                                uint32_t* slotOf, uint32_t* idOfSlot,
+                                const GatheredState* gathered) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
-  const bool rows = rows_walk_ok(P, gathered);
   if (B.poly)
-    launch_v3<true, true>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
+    launch_v3<true, true>(s, gathered, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                           cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
                           devFlags, slotOf, idOfSlot);
   else
-    launch_v3<true, false>(s, rows, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
+    launch_v3<true, false>(s, gathered, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                            cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
                            devFlags, slotOf, idOfSlot);
   return hipGetLastError();

@@ -43,8 +43,8 @@ enum : uint32_t {
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
-// Entries allocated past the last sphere in a system's sortedPos (read, and
-// discarded, by the row-range collide's batched loads).
+// Entries allocated past the last sphere in a system's sortedPos and
+// sortedQ (read, and discarded, by the row-range collide's batched loads).
 constexpr uint32_t kSortedPosPad = 8;
 
 // ---------------------------------------------------------------------------
@@ -99,6 +99,40 @@ int key_bits_for_cells(uint32_t numCells);
 // Number of scatter passes for `bits`-bit keys.
 int radix_passes(int bits, bool inPlace);
 
+// Frame of the 16-bit fixed-point sphere coordinates that the row-range
+// collide's candidate filter reads: u = rint((p - lo) * scale) clamped to
+// [0, 65535] per axis.  Clamping only shortens distances, so a sphere
+// outside the frame is still never dropped from a contact it has.
+struct QuantFrame {
+  float lo[3] = {0.0f, 0.0f, 0.0f};
+  float scale = 0.0f;
+};
+
+// The packed coordinates of one sphere: (x | y << 16, z).  A NaN
+// coordinate maps to 0.
+__device__ __forceinline__ uint2 quantize_pos(const QuantFrame& f, float x, float y, float z) {
+  auto q = [&](float p, float lo) {
+    return (uint32_t)fminf(fmaxf(rintf((p - lo) * f.scale), 0.0f), 65535.0f);
+  };
+  return make_uint2(q(x, f.lo[0]) | (q(y, f.lo[1]) << 16), q(z, f.lo[2]));
+}
+
+// Particle state gathered by the sort's last pass into sorted order: for
+// the output at position g with key h and value v, sortedPos[g] = pos[v]
+// and sortedVel[g] = (vel[v].xyz, bits of h); with idOfSlot (values are
+// slots) also indexOut[g] = idOfSlot[v]; with sortedQ also
+// sortedQ[g] = quantize_pos(frame, pos[v]).
+struct SortGather {
+  const float4* pos = nullptr;
+  const float4* vel = nullptr;
+  const uint32_t* idOfSlot = nullptr;
+  float4* sortedPos = nullptr;
+  float4* sortedVel = nullptr;
+  uint32_t* indexOut = nullptr;
+  uint2* sortedQ = nullptr;
+  QuantFrame frame;
+};
+
 // Stable ascending LSD sort of (key, value) pairs on the low `bits` bits.
 // valsIn == nullptr means the values are the identity 0..n-1.  keysIn may
 // equal keysOut and valsIn may equal valsOut (in-place); intermediate passes
@@ -109,19 +143,6 @@ int radix_passes(int bits, bool inPlace);
 // clearHash: when given, the first pass also resets to EMPTY every cell of
 // cellStart/cellEnd that starts a run in the sorted array clearHash (the
 // previous update's cells); requires a pass count >= 2 (minPasses).
-// Particle state gathered by the sort's last pass into sorted order: for
-// the output at position g with key h and value v, sortedPos[g] = pos[v]
-// and sortedVel[g] = (vel[v].xyz, bits of h); with idOfSlot (values are
-// slots) also indexOut[g] = idOfSlot[v].
-struct SortGather {
-  const float4* pos = nullptr;
-  const float4* vel = nullptr;
-  const uint32_t* idOfSlot = nullptr;
-  float4* sortedPos = nullptr;
-  float4* sortedVel = nullptr;
-  uint32_t* indexOut = nullptr;
-};
-
 hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFlags,
                             const uint32_t* keysIn, const uint32_t* valsIn,
                             uint32_t* keysOut, uint32_t* valsOut, uint32_t n, int bits,
@@ -200,14 +221,19 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
                           uint32_t* indexOut = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
 // posOut (optional): also scatter the sorted positions to id order.
-// gathered: sortedPos/sortedVel were written by a sort's SortGather into a
-// system's arrays (sortedVel.w holds the sorted hash, sortedPos carries
-// kSortedPosPad entries of padding), which enables the row-range walk.
+// Gathered state (the row-range walk): sortedPos/sortedVel/sortedQ were
+// written by a sort's SortGather into a system's arrays (sortedVel.w holds
+// the sorted hash, sortedPos and sortedQ carry kSortedPosPad entries of
+// padding, sortedQ's frame is `frame`).
+struct GatheredState {
+  const uint2* sortedQ = nullptr;
+  QuantFrame frame;
+};
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr,
-                          bool gathered = false);
+                          const GatheredState* gathered = nullptr);
 
 float contact_d2_threshold(float R);
 // collide + the next update's integrate/hash/histogram in one launch (see
@@ -219,7 +245,8 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* index, const uint32_t* cellStart,
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
-                                uint32_t* slotOf, uint32_t* idOfSlot, bool gathered);
+                                uint32_t* slotOf, uint32_t* idOfSlot,
+                                const GatheredState* gathered);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).
@@ -255,6 +282,7 @@ struct fc_system {
   float4* vel = nullptr;
   float4* sortedPos = nullptr;
   float4* sortedVel = nullptr;
+  uint2* sortedQ = nullptr;       // 16-bit coordinates of sortedPos (quant_frame())
   uint32_t* hash = nullptr;       // sorted hashes of the last update
   uint32_t* index = nullptr;      // sorted original indices of the last update
   uint32_t* keys = nullptr;       // unsorted hashes written by integrate+hash
@@ -277,4 +305,5 @@ struct fc_system {
   // Radii from pos.w (set by fc_system_set_polydisperse).
   int polydisperse = 0;
   fc::BoundaryArg boundary_arg() const;
+  fc::QuantFrame quant_frame() const;
 };

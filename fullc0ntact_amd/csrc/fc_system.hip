@@ -46,6 +46,7 @@ void free_system_buffers(fc_system* sys) {
   hipFree(sys->vel);
   hipFree(sys->sortedPos);
   hipFree(sys->sortedVel);
+  hipFree(sys->sortedQ);
   hipFree(sys->hash);
   hipFree(sys->index);
   hipFree(sys->keys);
@@ -54,6 +55,7 @@ void free_system_buffers(fc_system* sys) {
   hipFree(sys->cellStart);
   hipFree(sys->cellEnd);
   sys->pos = sys->vel = sys->sortedPos = sys->sortedVel = nullptr;
+  sys->sortedQ = nullptr;
   sys->hash = sys->index = sys->keys = sys->slotOf = sys->idOfSlot = nullptr;
   sys->cellStart = sys->cellEnd = nullptr;
 }
@@ -114,6 +116,8 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
   ga.vel = sys->vel;
   ga.sortedPos = sys->sortedPos;
   ga.sortedVel = sys->sortedVel;
+  ga.sortedQ = sys->sortedQ;
+  ga.frame = sys->quant_frame();
   if (n <= (uint32_t)RS_TILE) {
     if (timer) timer->begin(st, "sort_small");
     e = launch_sort_small(st, ctx->sort, sys->keys, nullptr, sys->hash, sys->index, n,
@@ -132,9 +136,9 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
     if (e != hipSuccess) return hip_error(e, "cell bounds");
   }
   if (timer) timer->begin(st, "collide");
+  const GatheredState gs{sys->sortedQ, ga.frame};
   e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
-                     sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr,
-                     /*gathered=*/true);
+                     sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, nullptr, &gs);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   sys->tablesTrackHash = true;
@@ -206,6 +210,8 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   ga.vel = sys->vel;
   ga.sortedPos = sys->sortedPos;
   ga.sortedVel = sys->sortedVel;
+  ga.sortedQ = sys->sortedQ;
+  ga.frame = sys->quant_frame();
   if (fromSlots) {
     ga.idOfSlot = sys->idOfSlot;
     ga.indexOut = sys->index;
@@ -232,15 +238,16 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     if (e != hipSuccess) return hip_error(e, "cell bounds");
   }
   if (timer) timer->begin(st, last ? "collide" : "collide_integrate");
+  const GatheredState gs{sys->sortedQ, ga.frame};
   if (last)
     e = launch_collide(st, P, sys->polydisperse != 0, ctx->devFlags, sys->vel, sys->sortedPos,
                        sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n,
-                       fromSlots ? sys->pos : nullptr, /*gathered=*/true);
+                       fromSlots ? sys->pos : nullptr, &gs);
   else
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
                              sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf,
-                             sys->idOfSlot, /*gathered=*/true);
+                             sys->idOfSlot, &gs);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
@@ -317,6 +324,17 @@ void generate_initial_state(float* pos, float* vel, uint32_t n, fc_particle_conf
 
 using namespace fc;
 
+// The region integrate keeps the spheres in: the box [-1, 1]^3 of the box
+// walls; with a distance map, x and y within the unit cylinder and z in
+// [-1, 10].  One scale for all axes keeps fixed-point distances isotropic.
+fc::QuantFrame fc_system::quant_frame() const {
+  fc::QuantFrame f;
+  const bool map = boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr;
+  f.lo[0] = f.lo[1] = f.lo[2] = -1.0f;
+  f.scale = 65535.0f / (map ? 11.0f : 2.0f);
+  return f;
+}
+
 fc::BoundaryArg fc_system::boundary_arg() const {
   fc::BoundaryArg b{};
   b.hasMap = (boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr) ? 1 : 0;
@@ -366,6 +384,7 @@ fc_status fc_system_create(fc_context* ctx, uint32_t numParticles, fc_uint3 grid
   if (e == hipSuccess) e = hipMalloc(&sys->vel, n * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&sys->sortedPos, (n + kSortedPosPad) * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&sys->sortedVel, n * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&sys->sortedQ, (n + kSortedPosPad) * sizeof(uint2));
   if (e == hipSuccess) e = hipMalloc(&sys->hash, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->index, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&sys->keys, n * sizeof(uint32_t));

@@ -330,10 +330,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         keysOut[g] = k;
         valsOut[g] = v;
         if (ga.pos && v < n) {  // v < n: values index the particle arrays
-          ga.sortedPos[g] = ga.pos[v];
+          const float4 pv = ga.pos[v];
+          ga.sortedPos[g] = pv;
           const float4 vv = ga.vel[v];
           ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
           if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
+          if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv.x, pv.y, pv.z);
         }
       }
     }
@@ -450,10 +452,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
       keysOut[g] = k;
       valsOut[g] = v;
       if (ga.pos && v < n) {
-        ga.sortedPos[g] = ga.pos[v];
+        const float4 pv = ga.pos[v];
+        ga.sortedPos[g] = pv;
         const float4 vv = ga.vel[v];
         ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
         if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
+        if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv.x, pv.y, pv.z);
       }
       const uint32_t kp = g > 0 ? s_keys[g - 1] : 0u;
       if (g == 0 || k != kp) {

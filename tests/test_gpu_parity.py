@@ -407,6 +407,70 @@ def test_collide_contact_distance_edges(gpu_ctx):
     assert np.any(nv[:, :3] != vel[:, :3])  # some pairs touch
 
 
+def _edge_pairs(rng, n_pairs, r, poly):
+    """Pairs of spheres at separations straddling their contact distance to
+    the last ulp (and some 0.01-0.5 % beyond it), spread over the whole box
+    with the first spheres of some pairs on the walls and in the corners;
+    pair centres are >= 3 contact distances apart, so pairs never meet."""
+    R = np.float32(2 * r)
+    lo, hi = -1.0 + r, 1.0 - r
+    ax = np.linspace(lo, hi, int((hi - lo) / (3.2 * R)) + 1)
+    lat = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    on_wall = np.any((lat == ax[0]) | (lat == ax[-1]), axis=1)
+    pick = np.concatenate([np.nonzero(on_wall)[0][:n_pairs // 3],
+                           rng.permutation(np.nonzero(~on_wall)[0])])[:n_pairs]
+    pos = np.zeros((2 * n_pairs, 4), np.float32)
+    for p, c in enumerate(lat[pick].astype(np.float32)):
+        ra, rb = (np.float32(rng.uniform(0.5, 1.0) * r), np.float32(rng.uniform(0.5, 1.0) * r)) \
+            if poly else (np.float32(r), np.float32(r))
+        d = np.float32(ra + rb)
+        if p % 5 == 4:
+            d = np.float32(d * (1.0 + rng.uniform(1e-4, 5e-3)))
+        else:
+            for _ in range(int(rng.integers(-4, 5))):
+                d = np.nextafter(d, np.float32(np.inf) if rng.random() < 0.5 else np.float32(0))
+        while True:  # a direction that keeps the partner inside the walls
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            b = (c + (u * d).astype(np.float32)).astype(np.float32)
+            if np.all(b >= lo) and np.all(b <= hi):
+                break
+        pos[2 * p, :3], pos[2 * p + 1, :3] = c, b
+        pos[2 * p, 3], pos[2 * p + 1, 3] = (ra, rb) if poly else (0.0, 0.0)
+    return pos
+
+
+@pytest.mark.parametrize("poly", [False, True])
+def test_system_collide_contact_distance_edges(gpu_ctx, poly):
+    # The system path's row-range collide drops candidates on 16-bit
+    # fixed-point coordinates before the exact test; no pair the reference
+    # counts as touching may be lost, anywhere in the box.  With no gravity
+    # and zero velocities the integrate leaves every position as placed.
+    from fullc0ntact_amd import ParticleSystem
+    n_pairs, grid, r = 2500, 64, 1.0 / 64.0
+    rng = np.random.default_rng(33 + int(poly))
+    pos = _edge_pairs(rng, n_pairs, r, poly)
+    n = len(pos)
+    vel = np.zeros((n, 4), np.float32)
+    P = sim_params(n, grid, r, gravity=(0.0, 0.0, 0.0), colliderPos=(50.0, 50.0, 50.0))
+    ps = ParticleSystem(n, (grid, grid, grid), context=gpu_ctx)
+    ps.m_params = P
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    if poly:
+        ps.setPolydisperse(True)
+    st = orc.State(pos, vel, P.numCells)
+    print(f"[edges] poly={poly} n={n}", flush=True)
+    ps.update(0.5)
+    orc.update_ex(st, 0.5, orc.params_from(P), None, poly=poly)
+    gp, gv = ps.getArray(ps.POSITION), ps.getArray(ps.VELOCITY)
+    assert np.array_equal(bits(gp), bits(st.pos4))
+    assert np.array_equal(bits(gv), bits(st.vel4))
+    touched = np.any(gv[:, :3] != 0.0, axis=1)
+    assert 0 < touched.sum() < n  # pairs on both sides of the contact distance
+    ps.close()
+
+
 def test_collide_list_overflow_path(gpu_ctx):
     # 40 overlapping spheres in one cell: every particle has more near
     # neighbours than the per-lane contact list holds.
