@@ -408,6 +408,54 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
       }
     }
     force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+  } else if (!split) {
+    // The list overflowed (a densely packed bed: up to ~60 contacts): walk
+    // the nine row ranges as one flattened sequence, kPairBatch candidates
+    // per iteration with their loads in flight together, each candidate
+    // through the exact pair force (non-touching ones add +0), the cell
+    // partial added when the sorted hash changes -- the list path's terms
+    // in the list path's order.
+    uint32_t B[9], Q[9];  // slots before row r; slot t of row r is sphere t + Q[r]
+    uint32_t total = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      B[r] = total;
+      Q[r] = lo[r] - total;  // modulo 2^32
+      total += len[r];
+    }
+    v3f cf = make_v3(0.0f, 0.0f, 0.0f);
+    uint32_t curHash = 0xffffffffu;
+#pragma unroll 1
+    for (uint32_t t = 0; t < total; t += kPairBatch) {
+      uint32_t jj[kPairBatch];
+      float4 qp[kPairBatch], qv[kPairBatch];
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        const uint32_t tu = t + (uint32_t)u < total ? t + (uint32_t)u : total - 1u;
+        uint32_t qr = Q[0];
+#pragma unroll
+        for (int r = 1; r < 9; ++r) qr = tu >= B[r] ? Q[r] : qr;
+        jj[u] = tu + qr;
+        qp[u] = sortedPos[jj[u]];
+        qv[u] = sortedVel[jj[u]];  // .w holds the sorted hash
+      }
+#pragma unroll
+      for (int u = 0; u < kPairBatch; ++u) {
+        if (t + (uint32_t)u < total && jj[u] != k) {
+          const uint32_t hu = __float_as_uint(qv[u].w);
+          if (hu != curHash) {
+            force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
+            cf = make_v3(0.0f, 0.0f, 0.0f);
+            curHash = hu;
+          }
+          v3f f = collide_spheres_fast(pos, xyz(qp[u]), vel, xyz(qv[u]), rA,
+                                       kPoly ? qp[u].w : rA, P.spring, P.damping, P.shear,
+                                       P.attraction);
+          cf = make_v3(cf.x + f.x, cf.y + f.y, cf.z + f.z);
+        }
+      }
+    }
+    force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
   } else {
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)

@@ -491,6 +491,57 @@ def test_collide_list_overflow_path(gpu_ctx):
     assert np.array_equal(bits(nv), bits(onv))
 
 
+def test_system_list_overflow_fallback(gpu_ctx):
+    # Clusters of 30-60 heavily overlapping spheres among loose ones: the
+    # clustered lanes hold more contacts than the contact list and take the
+    # row-range fallback, beside list-path lanes in the same waves.  Updates
+    # and a graph-launched run must equal the oracle bit for bit.
+    from fullc0ntact_amd import ParticleSystem
+    grid, r = 64, 1.0 / 64.0
+    rng = np.random.default_rng(44)
+    parts = [rng.uniform(-0.95, 0.95, size=(2000, 3))]
+    for _ in range(60):
+        centre = rng.uniform(-0.9, 0.9, 3)
+        parts.append(centre + rng.normal(scale=0.6 * r, size=(int(rng.integers(30, 61)), 3)))
+    xyz = np.concatenate(parts)
+    xyz = xyz[rng.permutation(len(xyz))]
+    n = len(xyz)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = np.clip(xyz, -1 + r, 1 - r)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.uniform(-0.01, 0.01, size=(n, 3))
+    P = sim_params(n, grid, r)
+    OP = orc.params_from(P)
+    ps = ParticleSystem(n, (grid, grid, grid), context=gpu_ctx)
+    ps.m_params = P
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    print(f"[overflow] n={n}", flush=True)
+    ps.update(0.5)
+    st.update(0.5, OP)
+    _compare_system(ps, st)
+    # the first update's collide saw the clusters whole
+    contacts = np.asarray(orc.contact_counts(st.sortedPos4, st.cellStart, st.cellEnd, OP))
+    assert contacts.max() > 40 and np.count_nonzero(contacts > 28) > 500
+    ps.update(0.5)
+    st.update(0.5, OP)
+    _compare_system(ps, st)
+    ps.close()
+    # the fused, graph-launched steps of a run from the same clusters
+    ps = ParticleSystem(n, (grid, grid, grid), context=gpu_ctx)
+    ps.m_params = P
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    ps.run(0.5, 11)
+    for _ in range(11):
+        st.update(0.5, OP)
+    assert np.array_equal(bits(ps.getArray(ps.POSITION)), bits(st.pos4))
+    assert np.array_equal(bits(ps.getArray(ps.VELOCITY)), bits(st.vel4))
+    ps.close()
+
+
 def test_collide_tiny_grid_visits_wrapped_cells_twice(gpu_ctx):
     # G = 2: cells x-1 and x+1 are the same cell, so the reference loop adds
     # that cell's forces twice; the fallback kernel must do exactly that.
