@@ -34,8 +34,10 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t b, uint32_t nb, u
 
 // XCD mapping of the collide's blocks (0 = contiguous chunks per XCD, else
 // runs of this many consecutive blocks per XCD).  On the settled 1M-sphere
-// bed, runs of 16 measured 118 us per launch, runs of 4 121 us, contiguous
-// chunks 136 us.
+// bed, an earlier kernel measured 118 us per launch with runs of 16, 121 us
+// with runs of 4 and 136 us with contiguous chunks; the row-range kernel
+// measured 97.2 us with runs of 16, 97.4 / 98.4 / 97.8 us with runs of
+// 8 / 32 / 64 and 103.1 us with the hardware's round-robin order.
 #ifndef FC_COLLIDE_XCD_GROUP
 #define FC_COLLIDE_XCD_GROUP 16
 #endif
