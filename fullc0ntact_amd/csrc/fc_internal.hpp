@@ -153,8 +153,10 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
 // cell-table work of a system step: resets the cells of clearHash, gathers
 // the state (gather), and writes cellStart/cellEnd of the sorted keys.
 // Bit-identical to radix_sort_pairs (with the same clearHash and gather)
-// followed by launch_cell_bounds.  The digit-0 histogram of keysIn is not
-// read; H[0] is zeroed for the next sort.
+// followed by launch_cell_bounds, for keys below numCells (the cells of a
+// system); it sorts on the keys' width, at most 8 * passes bits.  The
+// digit-0 histogram of keysIn is not read; H[0] is zeroed for the next
+// sort.
 hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t* keysIn,
                              const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
                              uint32_t n, int passes, const uint32_t* clearHash,

@@ -9,11 +9,13 @@
 
 namespace fc {
 
-// Block-wide exclusive scan of the values held by threads 0..255 (one per
-// digit).  Every thread of the block must call it (it synchronises); threads
-// from 256 on contribute nothing and get an unspecified result.
-// s_scratch holds 4 words.
-__device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t* s_scratch) {
+// Exclusive prefix sum of v over threads 0..kN-1 (kN a multiple of 64, at
+// most the block size); every thread of the block must call it.  Threads
+// from kN on contribute nothing and get an unspecified result.  s_scratch
+// holds kN / 64 words.
+template <int kN>
+__device__ __forceinline__ uint32_t scan_exclusive(uint32_t v, uint32_t* s_scratch) {
+  constexpr uint32_t kW = (uint32_t)kN / 64u;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   uint32_t inc = v;
@@ -22,12 +24,20 @@ __device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t* s_sc
     const uint32_t t = __shfl_up(inc, off);
     if (lane >= (uint32_t)off) inc += t;
   }
-  if (lane == 63u && wave < 4u) s_scratch[wave] = inc;
+  if (lane == 63u && wave < kW) s_scratch[wave] = inc;
   __syncthreads();
   uint32_t base = 0;
-  for (uint32_t w = 0; w < wave && w < 4u; ++w) base += s_scratch[w];
+  for (uint32_t w = 0; w < wave && w < kW; ++w) base += s_scratch[w];
   __syncthreads();
   return base + inc - v;
+}
+
+// Block-wide exclusive scan of the values held by threads 0..255 (one per
+// digit).  Every thread of the block must call it (it synchronises); threads
+// from 256 on contribute nothing and get an unspecified result.
+// s_scratch holds 4 words.
+__device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t* s_scratch) {
+  return scan_exclusive<256>(v, s_scratch);
 }
 
 // Histogram dataflow of the sort (no tickets, no fences, no last block):

@@ -67,11 +67,12 @@ __device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, u
   return flag | ((uint64_t)(epoch & kEpochMask) << 32) | (uint64_t)value;
 }
 
-// Lanes (among `active`) whose 8-bit digit equals this lane's digit.
+// Lanes (among `active`) whose kBits-bit digit equals this lane's digit.
+template <int kBits = RS_RADIX_BITS>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
   uint64_t peers = active;
 #pragma unroll
-  for (int b = 0; b < RS_RADIX_BITS; ++b) {
+  for (int b = 0; b < kBits; ++b) {
     const uint32_t bit = (d >> b) & 1u;
     const uint64_t m = __ballot(bit);
     peers &= bit ? m : ~m;
@@ -345,14 +346,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 
 // The whole sort of n <= RS_TILE keys in one block, for small systems whose
 // step would otherwise be four launches of one block each: every pass ranks
-// the keys in LDS exactly as k_rs_scatter ranks a tile (wave ballots,
-// wave-private digit counts, block scan), which in a single tile is the
-// global rank, so no histogram, look-back or further launch is needed.
+// the keys in LDS as k_rs_scatter ranks a tile (wave ballots, wave-private
+// digit counts, block scan), which in a single tile is the global rank, so
+// no histogram, look-back or further launch is needed; the digits are 9
+// bits wide (two passes for the 18-bit cells of a 64^3 grid instead of
+// three).
 // The block also does the other work of those launches: it resets the
 // cells of the previous sorted hashes (clearHash) and zeroes H[0] for the
 // next sort (as pass 1 does), gathers the state (SortGather) and writes the
 // cell bounds of the sorted keys (k_cell_bounds).  Outputs are bit-identical
 // to radix_sort_pairs + launch_cell_bounds.
+// The block holds every key, so its digits may be wider than the
+// multi-block passes' (kSmallBits): fewer passes over the same key width.
+constexpr int kSmallBits = 9;
+constexpr int kSmallRadix = 1 << kSmallBits;
+static_assert(kSmallRadix <= RS_THREADS && kSmallRadix % 64 == 0, "one digit per thread");
+
 __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int passes,
@@ -360,9 +369,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
     uint32_t* __restrict__ cellEnd, uint32_t numCells, SortGather ga, uint32_t* gHist) {
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
-  __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
-  __shared__ uint32_t s_tileStart[RS_RADIX];
-  __shared__ uint32_t s_scratch[RS_RADIX / 64];
+  __shared__ uint32_t s_waveHist[kWaves][kSmallRadix];
+  __shared__ uint32_t s_tileStart[kSmallRadix];
+  __shared__ uint32_t s_scratch[kSmallRadix / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t waveBase = wave * kWaveKeys;
   {
@@ -391,16 +400,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
     val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
   }
   for (int pass = 0; pass < passes; ++pass) {
-    const uint32_t shift = (uint32_t)pass * RS_RADIX_BITS;
-    for (int i = threadIdx.x; i < kWaves * RS_RADIX; i += RS_THREADS) (&s_waveHist[0][0])[i] = 0u;
+    const uint32_t shift = (uint32_t)pass * kSmallBits;
+    for (int i = threadIdx.x; i < kWaves * kSmallRadix; i += RS_THREADS) (&s_waveHist[0][0])[i] = 0u;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
       const uint32_t idx = waveBase + i * 64u + lane;
       const bool valid = idx < n;
       const uint64_t active = __ballot(valid);
-      const uint32_t d = (key[i] >> shift) & (RS_RADIX - 1);
-      const uint64_t peers = match_digit(d, active);
+      const uint32_t d = (key[i] >> shift) & (kSmallRadix - 1);
+      const uint64_t peers = match_digit<kSmallBits>(d, active);
       const uint32_t lower = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
       const uint32_t before = s_waveHist[wave][d];
       rank[i] = before + lower;
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
     }
     __syncthreads();
     const uint32_t d = threadIdx.x;
-    const bool digitThread = d < (uint32_t)RS_RADIX;
+    const bool digitThread = d < (uint32_t)kSmallRadix;
     uint32_t count = 0;
     if (digitThread) {
 #pragma unroll
@@ -418,14 +427,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
         count += c;
       }
     }
-    const uint32_t start = scan256_exclusive(digitThread ? count : 0u, s_scratch);
+    const uint32_t start = scan_exclusive<kSmallRadix>(digitThread ? count : 0u, s_scratch);
     if (digitThread) s_tileStart[d] = start;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
       const uint32_t idx = waveBase + i * 64u + lane;
       if (idx < n) {
-        const uint32_t dg = (key[i] >> shift) & (RS_RADIX - 1);
+        const uint32_t dg = (key[i] >> shift) & (kSmallRadix - 1);
         const uint32_t lp = s_tileStart[dg] + s_waveHist[wave][dg] + rank[i];
         s_keys[lp] = key[i];
         s_vals[lp] = val[i];
@@ -478,8 +487,13 @@ hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t*
                              const SortGather& gather) {
   if (n == 0) return hipSuccess;
   if (n > (uint32_t)RS_TILE || passes < 1 || passes > RS_MAX_PASSES) return hipErrorInvalidValue;
+  // The keys are cells, below numCells: their width in kSmallBits-bit
+  // digits (at least one pass, so that the sorted tile is staged).
+  int bits = key_bits_for_cells(numCells);
+  if (bits > passes * RS_RADIX_BITS) bits = passes * RS_RADIX_BITS;
+  const int smallPasses = bits > kSmallBits ? (bits + kSmallBits - 1) / kSmallBits : 1;
   hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(RS_THREADS), 0, stream, keysIn, valsIn, keysOut,
-                     valsOut, n, passes, clearHash, cellStart, cellEnd, numCells, gather,
+                     valsOut, n, smallPasses, clearHash, cellStart, cellEnd, numCells, gather,
                      s.histogram);
   return hipGetLastError();
 }
