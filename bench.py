@@ -5,14 +5,20 @@ lattice in [-1,1]^3, 256^3 hash grid (cell = 2r), spring .5, damping .02,
 shear .1, boundary damping -.5, gravity -0.0003, dt 0.5.  A step is one
 ParticleSystem::update: integrate -> hash -> sort -> reorder -> collide.
 
-Prints ONE JSON line (the last line of stdout) with the throughput in
-particle-steps/s of the multi-step run (graph-launched), the HBM roofline of
-the dominant kernel of that path (algorithmic bytes over its HIP-event-timed
-average duration; HBM traffic from the committed rocprofv3 counter summary)
-and the CPU baseline (the oracle, the CPU restatement of the same step,
-timed on this host).
+The lattice is first advanced --settle untimed steps (default 1500), by
+which time it has fallen and settled into a bed; then W warm-up steps, then
+K timed steps.  The timed region is K steps of the multi-step run
+(graph-launched) with the state resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Prints ONE JSON line (the last line of stdout) with the throughput in
+particle-steps/s, the HBM roofline of the dominant kernel of the timed path
+(algorithmic bytes of the reference stages it performs over its
+HIP-event-timed average duration; HBM traffic from the committed rocprofv3
+counter summary), the FP32-vs-FP64 force error on the timed state, the CPU
+baseline (the oracle, the CPU restatement of the same step, timed on this
+host's available threads and on one) and the surrogate-boundary leg.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--settle S] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -96,6 +102,21 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
     return None
 
 
+def survey_bytes(name: str, n: int, c_occ: int, c_touch: int) -> int | None:
+    """Algorithmic bytes of one launch by the reference-stage accounting:
+    integrate 64 N, calcHash 24 N, sort 52 N (one histogram read + three
+    passes of 8 B read and 8 B written), reorder 72 N + 16 C_occ, collide
+    52 N + 8 C_touch.  A fused launch counts every stage it performs."""
+    collide, integrate, calc_hash = 52 * n + 8 * c_touch, 64 * n, 24 * n
+    if name == "collide_integrate":
+        return collide + integrate + calc_hash
+    if name == "collide":
+        return collide
+    if name == "integrate_hash":
+        return integrate + calc_hash
+    return None
+
+
 def profiled_traffic(kernel: str) -> int | None:
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size)."""
@@ -110,7 +131,8 @@ def profiled_traffic(kernel: str) -> int | None:
     return None
 
 
-def gpu_leg(steps: int, warmup: int, timing_steps: int, progress_every: int = 0) -> dict:
+def gpu_leg(steps: int, warmup: int, timing_steps: int, settle: int,
+            progress_every: int = 0) -> dict:
     from fullc0ntact_amd import ParticleSystem, make_params
 
     ps = ParticleSystem(N, (GRID, GRID, GRID))
@@ -121,16 +143,19 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int, progress_every: int = 0)
     ps.m_params = make_params(N, GRID, RADIUS)
     ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
     ps.sync()
-    log(f"[bench] warmup {warmup} steps")
-    chunk = progress_every if progress_every > 0 else warmup
+    # The lattice falls, hits the floor and settles into a bed within about
+    # 1200 steps; the timed steps are taken from the settled bed so that the
+    # figure does not depend on how many warm-up steps the caller asks for.
+    log(f"[bench] settling the bed: {settle} untimed steps, then warmup {warmup} steps")
+    chunk = progress_every if progress_every > 0 else max(settle + warmup, 1)
     done = 0
-    while done < warmup:
-        k = min(chunk, warmup - done)
+    while done < settle + warmup:
+        k = min(chunk, settle + warmup - done)
         ps.run(DT, k)
         ps.sync()
         done += k
         if progress_every > 0:
-            log(f"[bench] warmup step {done}")
+            log(f"[bench] untimed step {done}")
     log(f"[bench] timed {steps} steps")
     t0 = time.perf_counter()
     ps.run(DT, steps)
@@ -155,16 +180,58 @@ def gpu_leg(steps: int, warmup: int, timing_steps: int, progress_every: int = 0)
     ps.sync()
     cs = ps.download("cellStart")
     c_occ, c_touch = cell_counts(cs, GRID)
+    state = (ps.getArray(ps.POSITION), ps.getArray(ps.VELOCITY))
     ps.close()
     return dict(elapsed=elapsed, run_stages=run_stages, update_stages=update_stages,
-                c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs)
+                c_occ=c_occ, c_touch=c_touch, copy_gbs=copy_gbs, state=state,
+                step_index=settle + warmup)
+
+
+def force_error(state, threads: int) -> dict:
+    """Error of the FP32 collide (bit-identical to the GPU's) against the
+    binary64 restatement of the same sum, on the bed state the GPU reached:
+    both over the same neighbour pairs (the cells are the FP32 ones)."""
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    P = orc.make_params(N, GRID, RADIUS)
+    pos, vel = state
+    h, i = orc.calc_hash(pos, P)
+    h, i = orc.sort_pairs(h, i)
+    cs, ce, sp, sv = orc.reorder(h, i, pos, vel, P.numCells)
+    f32 = orc.collide(sp, sv, i, cs, ce, P, newVel4=vel)[:, :3].astype(np.float64) - vel[:, :3]
+    f64 = orc.collide_f64(sp, sv, i, cs, ce, P) - vel[:, :3]
+    orc.set_threads(1)
+    d = np.abs(f32 - f64)
+    mag = np.linalg.norm(f64, axis=1)
+    touching = mag > 0
+    rel = np.linalg.norm(f32 - f64, axis=1)[touching] / mag[touching]
+    return {"max_abs_err_over_max_force": float(d.max() / np.abs(f64).max()),
+            "median_rel_err": float(np.median(rel)) if rel.size else 0.0,
+            "p99_rel_err": float(np.quantile(rel, 0.99)) if rel.size else 0.0,
+            "spheres_with_force": int(touching.sum()),
+            "what": "per-sphere force of one collide on the timed bed state: FP32 (the GPU's, "
+                    "bit-identical to the oracle's) against the binary64 restatement"}
+
+
+def host_threads() -> tuple[int, dict]:
+    """Host threads the CPU baseline may use: the CPUs this process may run
+    on, capped by OMP_NUM_THREADS where the environment sets it."""
+    avail = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    cap = int(env) if env and env.isdigit() and int(env) > 0 else avail
+    info = {"nproc": os.cpu_count(), "affinity_cpus": avail, "OMP_NUM_THREADS": env,
+            "model": cpu_model()}
+    return max(1, min(avail, cap)), info
 
 
 SDF_BODIES_LATTICE = 4        # 64 rigid bodies on a 4^3 lattice of [-1,1]^3
-SDF_SHAPES = 8                # seeded torus knots and superellipsoids
+SDF_SHAPES = None             # every body its own seeded shape and network
 SDF_SPHERES = 1 << 21         # 2,097,152 spheres
 SDF_RADIUS = 1.0 / 256
 SDF_MAP_CELLS = 256
+SDF_SAMPLES = 1 << 20         # exact-distance training samples per network
+SDF_EPOCHS = 30
+SDF_BATCH = 16384
 
 
 def sdf_leg(reps: int) -> dict:
@@ -177,10 +244,12 @@ def sdf_leg(reps: int) -> dict:
     from fullc0ntact_amd.boundary import build_scene
 
     ctx = _lib.Context()
-    log(f"[bench] sdf: building {SDF_SHAPES} shapes with {SDF_MAP_CELLS}^3 maps and networks")
-    sc = build_scene(ctx, lattice=SDF_BODIES_LATTICE, cells=SDF_MAP_CELLS, samples=1 << 16,
-                     epochs=30, seed=SEED, radius=SDF_RADIUS, shapes=SDF_SHAPES,
-                     knot_segments=(128, 8), ellipsoid_segments=(32, 16))
+    log(f"[bench] sdf: building {SDF_BODIES_LATTICE ** 3} shapes with {SDF_MAP_CELLS}^3 maps "
+        f"and networks on {SDF_SAMPLES} samples each")
+    sc = build_scene(ctx, lattice=SDF_BODIES_LATTICE, cells=SDF_MAP_CELLS, samples=SDF_SAMPLES,
+                     epochs=SDF_EPOCHS, batch=SDF_BATCH, seed=SEED, radius=SDF_RADIUS,
+                     shapes=SDF_SHAPES, knot_segments=(128, 8), ellipsoid_segments=(32, 16),
+                     log=lambda m: log(f"[bench] sdf: {m}"))
     log(f"[bench] sdf: maps {sc.build_s:.1f} s, training + calibration {sc.train_s:.1f} s")
     # spheres in the free space around the bodies: uniform draws whose
     # centre lies inside a body (map distance < 0) are dropped
@@ -222,8 +291,12 @@ def sdf_leg(reps: int) -> dict:
     log(f"[bench] sdf: map path {t_map:.1f} us, surrogate path {t_sdf:.1f} us "
         f"(bucket {ts['boundary_bucket']:.1f} + cull {ts['boundary_cull']:.1f} + band "
         f"{ts['boundary_exact_band']:.1f}), agreement {agreement:.6f}")
-    out = {"bodies": SDF_BODIES_LATTICE ** 3, "shapes": SDF_SHAPES, "spheres": SDF_SPHERES,
+    out = {"bodies": SDF_BODIES_LATTICE ** 3, "networks": len(sc.shape_maps),
+           "samples_per_network": SDF_SAMPLES, "epochs": SDF_EPOCHS, "spheres": SDF_SPHERES,
            "radius": SDF_RADIUS, "map_cells": SDF_MAP_CELLS,
+           "margins": "lattice bound (boundary.lattice_margins)",
+           "margin_range": [round(float(np.min(sc.margins)), 6),
+                            round(float(np.max(sc.margins)), 6)],
            "map_path_us": round(t_map, 2), "surrogate_path_us": round(t_sdf, 2),
            "bucket_us": round(ts["boundary_bucket"], 2),
            "cull_us": round(ts["boundary_cull"], 2),
@@ -238,6 +311,53 @@ def sdf_leg(reps: int) -> dict:
         m.close()
     ctx.close()
     return out
+
+
+BATCH_SCENES, BATCH_LO, BATCH_HI, BATCH_GRID = 256, 2000, 200000, 64
+
+
+def batch_leg(steps: int) -> dict:
+    """The batched ensemble: 256 independent scenes, sizes log-uniform in
+    [2000, 200000] (about 11M spheres), spring and damping jittered +-20%
+    per scene, r = 1/64 on a 64^3 grid each, one launch per stage with a
+    scene-segmented hash, graph-launched, launch shape autotuned on the
+    batch; plus the measured cost of a grid barrier in one persistent
+    (cooperative) launch against a kernel boundary in a graph."""
+    from fullc0ntact_amd import _lib, make_params
+    from fullc0ntact_amd.batch import Batch, ensemble_scene_counts, jittered_params
+    ctx = _lib.Context()
+    counts = ensemble_scene_counts(BATCH_SCENES, BATCH_LO, BATCH_HI, seed=SEED)
+    params = jittered_params(make_params(0, BATCH_GRID, 1.0 / 64), BATCH_SCENES, 0.2, seed=SEED)
+    for k, p in enumerate(params):
+        p.numBodies = int(counts[k])
+    b = Batch(counts, (BATCH_GRID,) * 3, context=ctx)
+    b.set_params(params)
+    b.reset(1, SEED, 0.1, 0.01)
+    log(f"[bench] batch: {BATCH_SCENES} scenes, {b.n} spheres; autotuning")
+    b.run(DT, 2)
+    best, ms = b.autotune(DT, 3)
+    b.run(DT, 3)  # graph capture with the chosen shape
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.run(DT, steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    e = b.stats_fp64()
+    assert np.all(np.isfinite(e))
+    n = b.n
+    b.close()
+    barrier_us, kernel_us = ctx.sync_probe(256, 1000)
+    ctx.close()
+    log(f"[bench] batch: {n * steps / el:.3e} particle-steps/s, {el / steps * 1e3:.3f} ms/step; "
+        f"grid barrier {barrier_us:.2f} us vs graph kernel boundary {kernel_us:.2f} us")
+    return {"scenes": BATCH_SCENES, "particles": int(n), "grid": [BATCH_GRID] * 3,
+            "steps": steps, "value": round(n * steps / el, 1), "unit": "particle-steps/s",
+            "ms_per_step": round(el / steps * 1e3, 4),
+            "tuning": {"integrate_per_thread": int(best.integratePerThread),
+                       "collide_block": int(best.collideBlock),
+                       "candidate_ms": [round(float(x), 4) for x in ms]},
+            "grid_barrier_us": round(barrier_us, 3),
+            "graph_kernel_boundary_us": round(kernel_us, 3)}
 
 
 def cpu_leg(target_seconds: float, threads: int) -> dict:
@@ -299,11 +419,15 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--settle", type=int, default=1500,
+                    help="untimed steps before the warmup, to reach the settled bed")
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-sdf", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--sdf-reps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--progress-every", type=int, default=0,
                     help="run the warmup in chunks of this many steps, logging each")
     a = ap.parse_args(argv)
@@ -312,7 +436,7 @@ def main(argv=None) -> int:
               "(this build has no multi-GPU path).", file=sys.stderr)
         return 2
 
-    g = gpu_leg(a.steps, a.warmup, a.timing_steps, a.progress_every)
+    g = gpu_leg(a.steps, a.warmup, a.timing_steps, a.settle, a.progress_every)
     ms = g["elapsed"] / a.steps * 1e3
     value = N * a.steps / g["elapsed"]
 
@@ -328,7 +452,8 @@ def main(argv=None) -> int:
 
     # The dominant kernel of the timed path: the largest total event time.
     dom = max(g["run_stages"].items(), key=lambda kv: kv[1][1])[0]
-    nbytes = stage_bytes(dom, N, g["c_occ"], g["c_touch"])
+    nbytes = survey_bytes(dom, N, g["c_occ"], g["c_touch"])
+    impl_bytes = stage_bytes(dom, N, g["c_occ"], g["c_touch"])
     roof = None
     if nbytes:
         us = run_us[dom]
@@ -336,17 +461,28 @@ def main(argv=None) -> int:
         traffic = profiled_traffic(dom)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dom, "bytes_per_launch": nbytes, "avg_launch_us": us,
+                "kernel": dom, "bytes_per_launch": nbytes,
+                "bytes_accounting": "reference stages fused in the launch: collide 52N + 8C_touch"
+                                    " + integrate 64N + calcHash 24N",
+                "implementation_bytes_per_launch": impl_bytes, "avg_launch_us": us,
                 "copy_probe_GBs": round(g["copy_gbs"], 1)}
+    threads, host = host_threads()
+    ferr = None
 
     sdf = None
     if not a.no_sdf:
         sdf = sdf_leg(a.sdf_reps)
+    batch = None
+    if not a.no_batch:
+        batch = batch_leg(a.batch_steps)
 
     cpu = None
     if not a.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        log(f"[bench] CPU baseline with {threads} threads")
+        # the CPU leg: the oracle's binary64 restatement states the FP32
+        # force error on the timed bed, then the CPU step is timed
+        log(f"[bench] FP32 vs FP64 force error on the timed bed ({threads} threads)")
+        ferr = force_error(g["state"], threads)
+        log(f"[bench] CPU baseline with {threads} threads on {host}")
         c = cpu_leg(a.cpu_seconds, threads)
         log("[bench] CPU baseline on one core")
         c1 = cpu_leg(a.cpu_seconds / 2, 1)
@@ -355,8 +491,9 @@ def main(argv=None) -> int:
         cpu = {"value": round(c["value"], 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
                "sample": f"{c['steps']} updates of the same 1,048,576-sphere lattice after one "
-                         f"warm-up update, {threads} OpenMP threads on {cpu_model()} "
+                         f"warm-up update, {threads} OpenMP threads on {host['model']} "
                          f"({c['seconds']:.1f} s)",
+               "host": host,
                "value_1_core": round(c1["value"], 1),
                "sample_1_core": f"{c1['steps']} updates on one thread ({c1['seconds']:.1f} s)",
                "small_case": small}
@@ -376,11 +513,13 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": "dem-1M-spheres", "particles": N, "grid": [GRID] * 3,
                    "radius": RADIUS, "dt": DT, "seed": SEED, "C_occ": g["c_occ"],
-                   "C_touch": g["c_touch"]},
+                   "C_touch": g["c_touch"], "timed_from_step": g["step_index"]},
+        "force_error_fp32_vs_fp64": ferr,
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_us": {"run": run_us, "update": upd_us},
         "sdf_vs_distance_map": sdf,
+        "batched_ensemble": batch,
     }
     print(json.dumps(line), flush=True)
     return 0

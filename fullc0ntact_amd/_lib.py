@@ -129,6 +129,8 @@ _SIGS = {
     "fc_reorder_data_and_find_cell_start": (
         _st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32]),
     "fc_collide": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32]),
+    "fc_eval_forces": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32]),
+    "fc_sync_probe": (_st, [_vp, _u32, C.c_int, _vp]),
     "fc_system_create": (_st, [_vp, _u32, UInt3, C.POINTER(_vp)]),
     "fc_system_destroy": (_st, [_vp]),
     "fc_system_num_particles": (_u32, [_vp]),
@@ -206,6 +208,7 @@ _SIGS = {
     "fc_sdf_set_params": (_st, [_vp, _vp]),
     "fc_sdf_loss_grad": (_st, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "fc_sdf_train": (_st, [_vp, _vp, _vp, _u32, _u32, _u32, C.c_float, C.c_float, _vp]),
+    "fc_sdf_map_margin": (_st, [_vp, _u32, _vp, C.c_float, C.c_float, _vp]),
     "fc_sdf_eval": (_st, [_vp, _vp, _vp, _u32, _vp, C.c_int]),
     "fc_mfma_probe": (_st, [_vp, _vp, _vp, _vp]),
     "fc_selftest_pair_math": (_st, [_vp, _vp]),
@@ -292,6 +295,13 @@ class Context:
         p = SimParams()
         check(lib.fc_get_parameters(self.handle, C.byref(p)))
         return p
+
+    def sync_probe(self, blocks: int = 256, iterations: int = 1000) -> tuple[float, float]:
+        """(us per grid-wide barrier in one cooperative launch, us per kernel
+        boundary in a captured graph), for `blocks` blocks of 256 threads."""
+        out = (C.c_float * 2)()
+        check(lib.fc_sync_probe(self.handle, int(blocks), int(iterations), out))
+        return float(out[0]), float(out[1])
 
     def copy_probe(self, nbytes: int, iterations: int = 20) -> float:
         """Average time (ms) of a plain device-to-device copy of nbytes."""

@@ -91,6 +91,7 @@ class Scene:
     shape_maps: list = None
     build_s: float = 0.0
     train_s: float = 0.0
+    margin_stats: list = None  # per shape, from lattice_margins
 
 
 def body_mesh(kind: str, rng, size: float):
@@ -109,13 +110,15 @@ def body_mesh(kind: str, rng, size: float):
 def build_scene(ctx: _lib.Context, lattice: int = 4, cells: int = 64, samples: int = 1 << 16,
                 epochs: int = 30, seed: int = 1973, domain=(-1.0, 1.0),
                 radius: float = 1.0 / 256, shapes: int | None = None,
-                knot_segments=(256, 12), ellipsoid_segments=(48, 24)) -> Scene:
+                knot_segments=(256, 12), ellipsoid_segments=(48, 24), batch: int = 1024,
+                margins: str = "lattice", log=None) -> Scene:
     """lattice^3 rigid bodies, one per lattice cell of [domain]^3, with
     random rotations, instancing `shapes` seeded shapes (torus knots and
     superellipsoids; default one shape per body): per shape a cells^3
     distance map over its box and a surrogate network trained on `samples`
-    exact mesh distances; per-body margins calibrated so that the culled
-    query keeps every contact of the exact one on held-out points."""
+    exact mesh distances (minibatches of `batch` per body); per-body margins
+    from the map lattice (lattice_margins, the default) or from held-out
+    points (calibrate_margins, margins="held_out")."""
     import time
     rng = np.random.default_rng(seed)
     lo, hi = domain
@@ -144,6 +147,8 @@ def build_scene(ctx: _lib.Context, lattice: int = 4, cells: int = 64, samples: i
         frames.append([*c, half])
     ctx.sync()
     build_s = time.perf_counter() - t0
+    if log:
+        log(f"{S} distance maps of {cells}^3 cells built in {build_s:.1f} s")
     frames = np.array(frames, np.float32)
     m2w, origins, body_net = [], [], []
     for b in range(B):
@@ -164,12 +169,62 @@ def build_scene(ctx: _lib.Context, lattice: int = 4, cells: int = 64, samples: i
         sd, _ = mesh_signed_distance(ctx, v, f, p)
         pts[s] = p
         lab[s] = sd
+    if log:
+        log(f"{S} x {samples} exact-distance labels in {time.perf_counter() - t0:.1f} s; "
+            f"training {S} networks, {epochs} epochs of minibatches of {batch}")
+    t1 = time.perf_counter()
     dp, dl = DeviceArray.from_host(ctx, pts), DeviceArray.from_host(ctx, lab)
-    sur.train(dp, dl, samples, 1024, epochs, 3e-3, 0.97)
-    shape_margins = calibrate_margins(ctx, sur, shape_maps, mesh_list, frames, radius, rng)
+    del pts, lab
+    sur.train(dp, dl, samples, batch, epochs, 3e-3, 0.97)
+    dp.free()
+    dl.free()
+    ctx.sync()
+    if log:
+        log(f"training {time.perf_counter() - t1:.1f} s")
+    t1 = time.perf_counter()
+    if margins == "lattice":
+        shape_margins, stats = lattice_margins(sur, shape_maps, frames, radius)
+    else:
+        shape_margins = calibrate_margins(ctx, sur, shape_maps, mesh_list, frames, radius, rng)
+        stats = None
+    ctx.sync()
+    if log:
+        log(f"margins ({margins}) {time.perf_counter() - t1:.1f} s")
     train_s = time.perf_counter() - t0
-    margins = shape_margins[np.array(body_net)]
-    return Scene(bodies, sur, margins, mesh_list, maps, frames, shape_maps, build_s, train_s)
+    body_margins = shape_margins[np.array(body_net)]
+    return Scene(bodies, sur, body_margins, mesh_list, maps, frames, shape_maps, build_s, train_s,
+                 stats)
+
+
+def lattice_margins(sur: SdfSurrogate, maps, frames, radius: float, safety: float = 1.25):
+    """Per shape k, a margin m_k with which the cull (network distance y <=
+    radius + m_k) keeps every sphere whose map distance is below radius.
+
+    The map distance d_map at a query q is the trilinear mix of its cell's
+    corner values.  Along each axis its slope is a weighted mean of corner
+    differences over one lattice spacing h, at most 1 because the vertex
+    values are exact signed distances (1-Lipschitz); so from q to the
+    nearest corner v (at most h/2 per axis) d_map changes by at most 1.5 h,
+    and a contact (d_map(q) < r) has d(v) < r + 1.5 h.  Then
+      y(q) - r <= (y(v) - r) + L |q - v| <= over + L h sqrt(3) / 2,
+    where `over` is the largest y(v) - r over the vertices with
+    d(v) < r + 1.5 h, evaluated by the cull's own BF16 inference
+    (fc_sdf_map_margin), and L bounds the network's slope.  L is taken as
+    `safety` times the largest secant slope along the lattice's x, y and z
+    edges at that band: the margin is a bound as long as the network does
+    not vary inside a lattice cell by more than `safety` times its steepest
+    sampled edge.  A small absolute term covers the float rounding of the
+    map query."""
+    out = np.zeros(len(maps), np.float32)
+    stats = []
+    for k, m in enumerate(maps):
+        h = float(m.cell_size)
+        st = sur.map_margin(k, m, radius, 1.5 * h)
+        lip = safety * st["slope"]
+        out[k] = max(st["over"], 0.0) + lip * h * float(np.sqrt(3.0)) / 2.0 \
+            + 1e-6 * float(frames[k, 3])
+        stats.append(dict(st, h=h, margin=float(out[k])))
+    return out, stats
 
 
 def calibrate_margins(ctx, sur: SdfSurrogate, maps, mesh_list, frames, radius: float, rng,

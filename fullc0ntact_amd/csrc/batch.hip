@@ -19,6 +19,17 @@
 #include <cstring>
 #include <vector>
 
+namespace fc {
+// Per-scene constants of the row-range collide (the single system computes
+// them per launch): exact reciprocal cell sizes (or zeros), the contact
+// pre-test bound on d^2 and the fixed-point candidate filter's bound.
+struct SceneCollide {
+  v3f invCell;
+  float contactD2;
+  uint32_t qThreshold;
+};
+}  // namespace fc
+
 struct fc_batch {
   fc_context* ctx = nullptr;
   uint32_t S = 0;
@@ -31,6 +42,8 @@ struct fc_batch {
   uint32_t* dOffsets = nullptr;
   fc_sim_params* dParams = nullptr;
   float* dContactD2 = nullptr;
+  fc::SceneCollide* dSceneCollide = nullptr;
+  uint2* sortedQ = nullptr;  // 16-bit coordinates of sortedPos (the box frame)
   float4* pos = nullptr;
   float4* vel = nullptr;
   float4* sortedPos = nullptr;
@@ -130,10 +143,42 @@ __global__ __launch_bounds__(BS) void k_batch_collide(
   const fc_sim_params P = prm[s];
   const size_t base = (size_t)s * C;
   const float4 p4 = sortedPos[k];
-  const v3f nv = collide_one_v3<false, BS>(P, contactD2[s], k, xyz(p4), xyz(sortedVel[k]),
-                                           P.particleRadius, sortedPos, sortedVel,
-                                           cellStart + base, cellEnd + base, n, s_list);
-  newVel[index[k]] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+  const v3f pos = xyz(p4), vel = xyz(sortedVel[k]);
+  const v3f force = add_collider(
+      P, pos, vel, P.particleRadius,
+      collide_one_v3<false, BS>(P, contactD2[s], k, pos, vel, P.particleRadius, sortedPos,
+                                sortedVel, cellStart + base, cellEnd + base, n, s_list));
+  newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
+}
+
+// collideD per scene on the arrays gathered by the sort (the row-range walk
+// of the single system, collide_one_rows): the scene comes from the sorted
+// key in sortedVel.w, the cell tables are the scene's slice of the batch
+// tables, the parameters and row-walk constants the scene's own.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_collide_rows(
+    const fc_sim_params* __restrict__ prm, const SceneCollide* __restrict__ sc, int cellBits,
+    uint32_t C, float4* __restrict__ newVel, const float4* __restrict__ sortedPos,
+    const float4* __restrict__ sortedVel, const uint2* __restrict__ sortedQ,
+    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
+    const uint32_t* __restrict__ cellEnd, uint32_t n) {
+  __shared__ uint32_t s_list[kListCap + 1][BS];
+  const uint32_t k =
+      xcd_grouped_block(blockIdx.x, gridDim.x, kCollideXcdGroup * (256 / BS)) * BS + threadIdx.x;
+  if (k >= n) return;
+  const float4 v4 = sortedVel[k];
+  const uint32_t s = __float_as_uint(v4.w) >> cellBits;
+  const fc_sim_params P = prm[s];
+  const SceneCollide c = sc[s];
+  const size_t base = (size_t)s * C;
+  const float4 p4 = sortedPos[k];
+  const v3f pos = xyz(p4), vel = xyz(v4);
+  const v3f force = add_collider(
+      P, pos, vel, P.particleRadius,
+      collide_one_rows<false, BS>(P, c.invCell, c.contactD2, k, pos, vel, P.particleRadius,
+                                  sortedPos, sortedVel, sortedQ, c.qThreshold, cellStart + base,
+                                  cellEnd + base, n, s_list));
+  newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
 // Kinetic energy and momentum per scene in FP64, one block per scene, fixed
@@ -194,8 +239,37 @@ hipError_t launch_batch_collide(fc_batch* b) {
   return hipGetLastError();
 }
 
+hipError_t launch_batch_collide_rows(fc_batch* b) {
+  hipStream_t st = b->ctx->stream;
+  const int bs = b->tuning.collideBlock;
+#define FC_BCR(BS)                                                                             \
+  hipLaunchKernelGGL(k_batch_collide_rows<BS>, dim3(blocks_for(b->n, BS)), dim3(BS), 0, st,   \
+                     b->dParams, b->dSceneCollide, b->cellBits, b->C, b->vel, b->sortedPos,   \
+                     b->sortedVel, b->sortedQ, b->index, b->cellStart, b->cellEnd, b->n)
+  if (bs == 64) FC_BCR(64);
+  else if (bs == 128) FC_BCR(128);
+  else FC_BCR(256);
+#undef FC_BCR
+  return hipGetLastError();
+}
+
 // S * C < 2^32 is checked at creation.
 int batch_key_bits(const fc_batch* b) { return key_bits_for_cells(b->S * b->C); }
+
+// The row-range walk needs every grid dimension >= 4 (27 distinct cells,
+// rows of three) and keys below 2^30 (its "no cell" marker).
+bool batch_rows_ok(const fc_batch* b) {
+  return b->grid.x >= 4 && b->grid.y >= 4 && b->grid.z >= 4 &&
+         (uint64_t)b->S * b->C <= (1ull << 30);
+}
+
+// Frame of the fixed-point coordinates: the box walls at +-1 of every scene.
+QuantFrame batch_frame() {
+  QuantFrame f;
+  f.lo[0] = f.lo[1] = f.lo[2] = -1.0f;
+  f.scale = 65535.0f / 2.0f;
+  return f;
+}
 
 void drop_batch_graph(fc_batch* b) {
   if (b->graphExec) hipGraphExecDestroy(b->graphExec);
@@ -219,11 +293,29 @@ fc_status enqueue_batch_update(fc_batch* b, float dt) {
   if (b->n == 0) return FC_SUCCESS;
   const int bits = batch_key_bits(b);
   FC_HIP_TRY(launch_batch_integrate_hash(b, dt, prev, radix_passes(bits, false)));
-  FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash, b->index,
-                              b->n, bits, nullptr, /*histogramReady=*/true));
-  FC_HIP_TRY(launch_reorder(st, b->cellStart, b->cellEnd, b->sortedPos, b->sortedVel, b->hash,
-                            b->index, b->pos, b->vel, b->n, (uint32_t)cells));
-  FC_HIP_TRY(launch_batch_collide(b));
+  if (batch_rows_ok(b)) {
+    // The sort's last pass gathers the state into sorted order (the key
+    // bits in sortedVel.w, the fixed-point coordinates), a small kernel
+    // writes the cell tables, and the row-range collide runs per scene.
+    SortGather ga;
+    ga.pos = b->pos;
+    ga.vel = b->vel;
+    ga.sortedPos = b->sortedPos;
+    ga.sortedVel = b->sortedVel;
+    ga.sortedQ = b->sortedQ;
+    ga.frame = batch_frame();
+    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash,
+                                b->index, b->n, bits, nullptr, /*histogramReady=*/true, 1,
+                                nullptr, nullptr, nullptr, &ga));
+    FC_HIP_TRY(launch_cell_bounds(st, b->cellStart, b->cellEnd, b->hash, b->n, (uint32_t)cells));
+    FC_HIP_TRY(launch_batch_collide_rows(b));
+  } else {
+    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash,
+                                b->index, b->n, bits, nullptr, /*histogramReady=*/true));
+    FC_HIP_TRY(launch_reorder(st, b->cellStart, b->cellEnd, b->sortedPos, b->sortedVel, b->hash,
+                              b->index, b->pos, b->vel, b->n, (uint32_t)cells));
+    FC_HIP_TRY(launch_batch_collide(b));
+  }
   b->tablesTrackHash = true;
   return FC_SUCCESS;
 }
@@ -252,8 +344,16 @@ fc_status capture_batch_graph(fc_batch* b, float dt) {
 
 fc_status upload_batch_params(fc_batch* b) {
   std::vector<float> d2(b->S);
-  for (uint32_t s = 0; s < b->S; ++s)
-    d2[s] = contact_d2_threshold(b->params[s].particleRadius + b->params[s].particleRadius);
+  std::vector<SceneCollide> sc(b->S);
+  for (uint32_t s = 0; s < b->S; ++s) {
+    const fc_sim_params& P = b->params[s];
+    d2[s] = contact_d2_threshold(P.particleRadius + P.particleRadius);
+    sc[s].invCell = inv_cell(P);
+    sc[s].contactD2 = d2[s];
+    sc[s].qThreshold = quant_threshold(P.particleRadius + P.particleRadius, batch_frame().scale);
+  }
+  FC_HIP_TRY(hipMemcpyAsync(b->dSceneCollide, sc.data(), b->S * sizeof(SceneCollide),
+                            hipMemcpyHostToDevice, b->ctx->stream));
   FC_HIP_TRY(hipMemcpyAsync(b->dParams, b->params.data(), b->S * sizeof(fc_sim_params),
                             hipMemcpyHostToDevice, b->ctx->stream));
   FC_HIP_TRY(hipMemcpyAsync(b->dContactD2, d2.data(), b->S * sizeof(float),
@@ -267,6 +367,8 @@ void free_batch(fc_batch* b) {
   hipFree(b->dOffsets);
   hipFree(b->dParams);
   hipFree(b->dContactD2);
+  hipFree(b->dSceneCollide);
+  hipFree(b->sortedQ);
   hipFree(b->pos);
   hipFree(b->vel);
   hipFree(b->sortedPos);
@@ -325,9 +427,12 @@ fc_status fc_batch_create(fc_context* ctx, uint32_t numScenes, const uint32_t* s
   hipError_t e = hipMalloc(&b->dOffsets, (numScenes + 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->dParams, numScenes * sizeof(fc_sim_params));
   if (e == hipSuccess) e = hipMalloc(&b->dContactD2, numScenes * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&b->dSceneCollide, numScenes * sizeof(SceneCollide));
   if (e == hipSuccess) e = hipMalloc(&b->pos, n * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&b->vel, n * sizeof(float4));
-  if (e == hipSuccess) e = hipMalloc(&b->sortedPos, n * sizeof(float4));
+  // sortedPos and sortedQ carry the row walk's padding past the last sphere
+  if (e == hipSuccess) e = hipMalloc(&b->sortedPos, (n + kSortedPosPad) * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&b->sortedQ, (n + kSortedPosPad) * sizeof(uint2));
   if (e == hipSuccess) e = hipMalloc(&b->sortedVel, n * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc(&b->hash, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->index, n * sizeof(uint32_t));

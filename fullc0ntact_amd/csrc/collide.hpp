@@ -67,6 +67,16 @@ static_assert(kRowsBatch >= 1 && (uint32_t)kRowsBatch - 1u <= kSortedPosPad, "ro
 // Candidates whose loads are issued together in k_collide_v3.
 constexpr int kBatch = 4;
 
+// The cursor sphere of collideD (particles_kernel.cu:286-288): its pair
+// force (at rest, no attraction) added to the 27 cells' sum.
+__device__ __forceinline__ v3f add_collider(const fc_sim_params& P, v3f pos, v3f vel, float rA,
+                                            v3f force) {
+  const v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
+  const v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), rA, P.colliderRadius,
+                                P.spring, P.damping, P.shear, 0.0f);
+  return make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
+}
+
 // One lane per sorted particle, two phases.
 //  A: for each of the 9 neighbour rows, the candidates of its 3 cells are
 //     walked as one flattened sequence; a candidate whose squared distance
@@ -81,6 +91,8 @@ constexpr int kBatch = 4;
 //     so the result is bit-identical to the 27-cell loop, while the wave
 //     spends the expensive pair arithmetic only on (near-)contacts.
 // A lane whose list would overflow falls back to the direct 27-cell loop.
+// Returns the sum of the 27 cells' partials (the caller adds the cursor
+// sphere, add_collider, and the velocity).
 // kPoly: radii come from pos.w (rA for this sphere); contactD2 is then the
 // threshold of the largest contact distance 2 * particleRadius, which
 // bounds every rA + rB.
@@ -201,11 +213,7 @@ __device__ __forceinline__ v3f collide_one_v3(const fc_sim_params& P, float cont
           force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
         }
   }
-  v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
-  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), rA, P.colliderRadius,
-                          P.spring, P.damping, P.shear, 0.0f);
-  force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
-  return make_v3(vel.x + force.x, vel.y + force.y, vel.z + force.z);
+  return force;
 }
 
 // Load through a 32-bit byte offset from a uniform base pointer, so that the
@@ -256,7 +264,7 @@ __device__ __forceinline__ uint32_t quant_d2(uint2 a, uint2 b) {
 // dimension is >= 3), adding each cell's partial to the total in cell
 // order.  As in collide_one_v3 every dropped or non-touching candidate
 // contributed exactly +0, so the result is bit-identical to the 27-cell
-// loop.
+// loop.  Returns the 27 cells' sum, as collide_one_v3 does.
 template <bool kPoly, int BS>
 __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invCell,
                                                 float contactD2, uint32_t k, v3f pos, v3f vel,
@@ -478,11 +486,7 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
           force = make_v3(force.x + cf.x, force.y + cf.y, force.z + cf.z);
         }
   }
-  v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
-  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), rA, P.colliderRadius,
-                          P.spring, P.damping, P.shear, 0.0f);
-  force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
-  return make_v3(vel.x + force.x, vel.y + force.y, vel.z + force.z);
+  return force;
 }
 
 }  // namespace fc

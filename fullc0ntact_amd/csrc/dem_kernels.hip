@@ -204,11 +204,14 @@ __device__ __forceinline__ v3f cell_sum(const fc_sim_params& P, v3f pos, v3f vel
 // chain of 9-18 dependent loads.  Empty cells contribute nothing: their
 // partial sum is +0 and the running total is never -0, so skipping the add
 // leaves every bit unchanged.
+// forceOut != nullptr is evalForces_launcher (particledem_um.cu:538-589):
+// no cursor sphere, and the force also stored as float3 at the original
+// index.
 __global__ __launch_bounds__(kBlock) void k_collide_v2(
     fc_sim_params P, float4* __restrict__ newVel, const float4* __restrict__ sortedPos,
     const float4* __restrict__ sortedVel, const uint32_t* __restrict__ index,
     const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n,
-    int poly, uint32_t* devFlags) {
+    int poly, uint32_t* devFlags, float* __restrict__ forceOut) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
   const float4 p4 = sortedPos[k];
@@ -238,11 +241,14 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
       }
     }
   }
-  v3f cpos = make_v3(P.colliderPos.x, P.colliderPos.y, P.colliderPos.z);
-  v3f f = collide_spheres(pos, cpos, vel, make_v3(0.0f, 0.0f, 0.0f), r, P.colliderRadius,
-                          P.spring, P.damping, P.shear, 0.0f);
-  force = make_v3(force.x + f.x, force.y + f.y, force.z + f.z);
   const uint32_t orig = index[k];
+  if (forceOut == nullptr) {
+    force = add_collider(P, pos, vel, r, force);
+  } else {  // evalForces: no cursor sphere, the force stored as float3
+    forceOut[3 * (size_t)orig] = force.x;
+    forceOut[3 * (size_t)orig + 1] = force.y;
+    forceOut[3 * (size_t)orig + 2] = force.z;
+  }
   newVel[orig] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
@@ -264,6 +270,9 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
 // kRows selects the row-range neighbour walk (collide_one_rows), which
 // needs the sorted hashes in sortedVel.w and the fixed-point coordinates
 // sortedQ with their filter bound qThreshold.
+// forceOut != nullptr (not fused) is evalForces_launcher
+// (particledem_um.cu:538-589): no cursor sphere, and the force also stored
+// as float3 at the original index.
 template <bool kFuse, bool kPoly, bool kRows>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
     fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
@@ -272,7 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const uint32_t* __restrict__ cellEnd, uint32_t n, uint32_t xcdGroup, float4* __restrict__ pos,
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
     uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
-    v3f invCell, const uint2* __restrict__ sortedQ, uint32_t qThreshold) {
+    v3f invCell, const uint2* __restrict__ sortedQ, uint32_t qThreshold,
+    float* __restrict__ forceOut) {
   __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
@@ -286,14 +296,22 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     const float4 p4 = sortedPos[k];
     const float rA = kPoly ? p4.w : P.particleRadius;
     if (kPoly && !(rA <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
-    const v3f nv =
-        kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, xyz(p4),
-                                                xyz(sortedVel[k]), rA, sortedPos, sortedVel,
-                                                sortedQ, qThreshold, cellStart, cellEnd, n,
-                                                s_list)
-              : collide_one_v3<kPoly, kBlock>(P, contactD2, k, xyz(p4), xyz(sortedVel[k]), rA,
-                                              sortedPos, sortedVel, cellStart, cellEnd, n, s_list);
+    const v3f pos0 = xyz(p4), vel0 = xyz(sortedVel[k]);
+    v3f force =
+        kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, pos0, vel0, rA,
+                                                sortedPos, sortedVel, sortedQ, qThreshold,
+                                                cellStart, cellEnd, n, s_list)
+              : collide_one_v3<kPoly, kBlock>(P, contactD2, k, pos0, vel0, rA, sortedPos,
+                                              sortedVel, cellStart, cellEnd, n, s_list);
     const uint32_t orig = index[k];
+    if (kFuse || forceOut == nullptr) {
+      force = add_collider(P, pos0, vel0, rA, force);
+    } else {  // evalForces: no cursor sphere, the force stored as float3
+      forceOut[3 * (size_t)orig] = force.x;
+      forceOut[3 * (size_t)orig + 1] = force.y;
+      forceOut[3 * (size_t)orig + 2] = force.z;
+    }
+    const v3f nv = make_v3(vel0.x + force.x, vel0.y + force.y, vel0.z + force.z);
     if (!kFuse) {
       newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
       if (pos) pos[orig] = p4;
@@ -417,7 +435,7 @@ static float pow2_reciprocal(float c) {
 }
 
 // Reciprocal cell sizes for cell_quotients: all three or none.
-static v3f inv_cell(const fc_sim_params& P) {
+v3f inv_cell(const fc_sim_params& P) {
   const float ix = pow2_reciprocal(P.cellSize.x), iy = pow2_reciprocal(P.cellSize.y),
               iz = pow2_reciprocal(P.cellSize.z);
   if (ix == 0.0f || iy == 0.0f || iz == 0.0f) return v3f{0.0f, 0.0f, 0.0f};
@@ -432,7 +450,7 @@ static v3f inv_cell(const fc_sim_params& P) {
 // 2.5 units, which also covers the rounding of the reference's own distance
 // (~1e-7 relative).  Every pair is a candidate when the bound leaves the
 // int16 range of the coordinate differences.
-static uint32_t quant_threshold(float R, float scale) {
+uint32_t quant_threshold(float R, float scale) {
   const double b = (double)R * (double)scale + 2.5;
   if (!(b < 32000.0)) return 0xffffffffu;
   return (uint32_t)std::ceil(b * b);
@@ -445,25 +463,27 @@ static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                       const float4* sortedVel, const uint32_t* index, const uint32_t* cellStart,
                       const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
-                      uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot) {
+                      uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
+                      float* forceOut = nullptr) {
   if (gs != nullptr && rows_walk_ok(P))
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
                        inv_cell(P), gs->sortedQ,
-                       quant_threshold(P.particleRadius + P.particleRadius, gs->frame.scale));
+                       quant_threshold(P.particleRadius + P.particleRadius, gs->frame.scale),
+                       forceOut);
   else
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       inv_cell(P), nullptr, 0u);
+                       inv_cell(P), nullptr, 0u, forceOut);
 }
 
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut,
-                          const GatheredState* gathered) {
+                          const GatheredState* gathered, float* forceOut) {
   if (n == 0) return hipSuccess;
   dim3 g(grid_for(n, kBlock)), b(kBlock);
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
@@ -472,17 +492,17 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
   if (n >= (1u << 27) || P.gridSize.x < 4) {
     if (posOut) return hipErrorInvalidValue;  // only the multi-step run asks for positions
     hipLaunchKernelGGL(k_collide_v2, g, b, 0, s, P, newVel, sortedPos, sortedVel, index,
-                       cellStart, cellEnd, n, poly ? 1 : 0, devFlags);
+                       cellStart, cellEnd, n, poly ? 1 : 0, devFlags, forceOut);
     return hipGetLastError();
   }
   if (poly)
     launch_v3<false, true>(s, gathered, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                            index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                           nullptr, devFlags, nullptr, nullptr);
+                           nullptr, devFlags, nullptr, nullptr, forceOut);
   else
     launch_v3<false, false>(s, gathered, g, b, P, BoundaryArg{}, d2, newVel, sortedPos, sortedVel,
                             index, cellStart, cellEnd, n, posOut, 0.0f, nullptr, 0, nullptr,
-                            nullptr, devFlags, nullptr, nullptr);
+                            nullptr, devFlags, nullptr, nullptr, forceOut);
   return hipGetLastError();
 }
 

@@ -399,6 +399,22 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
   return FC_SUCCESS;
 }
 
+fc_status fc_eval_forces(fc_context* ctx, float* newVel, float* force, const float* sortedPos,
+                         const float* sortedVel, const uint32_t* index, const uint32_t* cellStart,
+                         const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells) {
+  if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
+  if (numParticles == 0) return FC_SUCCESS;
+  if (!newVel || !force || !sortedPos || !sortedVel || !index || !cellStart || !cellEnd)
+    return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  if (numCells != ctx->params.numCells)
+    return set_error(FC_ERROR_INVALID_VALUE, "numCells differs from the context parameters");
+  hipError_t e = launch_collide(ctx->stream, ctx->params, false, ctx->devFlags, (float4*)newVel,
+                                (const float4*)sortedPos, (const float4*)sortedVel, index,
+                                cellStart, cellEnd, numParticles, nullptr, nullptr, force);
+  if (e != hipSuccess) return hip_error(e, "evalForces");
+  return FC_SUCCESS;
+}
+
 fc_status fc_context_enable_timing(fc_context* ctx, int enable) {
   if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
   ctx->timer.enabled = enable != 0;

@@ -223,6 +223,8 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
                           uint32_t* indexOut = nullptr);
 // poly: radii from pos.w (bounded by P.particleRadius, else DEVERR_RADIUS).
 // posOut (optional): also scatter the sorted positions to id order.
+// forceOut (optional): evalForces -- no cursor sphere, and the 27 cells'
+// force stored as float3 at the original index.
 // Gathered state (the row-range walk): sortedPos/sortedVel/sortedQ were
 // written by a sort's SortGather into a system's arrays (sortedVel.w holds
 // the sorted hash, sortedPos and sortedQ carry kSortedPosPad entries of
@@ -235,9 +237,15 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
                           float4* newVel, const float4* sortedPos, const float4* sortedVel,
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr,
-                          const GatheredState* gathered = nullptr);
+                          const GatheredState* gathered = nullptr, float* forceOut = nullptr);
 
 float contact_d2_threshold(float R);
+// Exact reciprocals of the cell sizes when all three are powers of two (the
+// multiply then equals the division), else zeros (cell_quotients divides).
+v3f inv_cell(const fc_sim_params& P);
+// Bound on the squared fixed-point distance (units of 1/scale) of every pair
+// closer than R, for the row-range collide's candidate filter.
+uint32_t quant_threshold(float R, float scale);
 // collide + the next update's integrate/hash/histogram in one launch (see
 // dem_kernels.hip); usable when collide_can_fuse(P, n).
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n);

@@ -889,6 +889,97 @@ __global__ __launch_bounds__(kTB) void k_boundary_exact(
   }
 }
 
+// Margin statistics of one network against one distance map, over the map's
+// lattice vertices (the points where the map holds exact distances).  The
+// lattice is walked in blocks of 4 x 2 x 2 vertices, one 16-vertex MFMA
+// tile each (lane column c holds vertex (c & 3, (c >> 2) & 1, c >> 3) of the
+// block); blocks with no vertex within radius + band of the surface are
+// skipped.  The BF16 network is evaluated, as the cull evaluates it, and for
+// the vertices v with d(v) < radius + band the kernel takes
+//   stats[0] = max (y(v) - radius), stats[3] = max |y(v) - d(v)|,
+//   stats[2] = the number of such vertices,
+//   stats[1] = max |y(v') - y(v)| / |v' - v| over the x, y and z neighbours
+//              v' of v in the same block (the network's largest secant
+//              slope along the lattice edges at the band),
+// as order-preserving integer keys of the floats (stats[2] a plain count).
+// stats must start at {key(-inf), 0, 0, 0}.
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(kTB) void k_sdf_map_margin(
+    const uint16_t* __restrict__ packed, const float* __restrict__ params,
+    const float* __restrict__ freq, const float* __restrict__ frame, uint32_t net,
+    DistMapView m, float radius, float band, uint32_t* __restrict__ stats) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const uint32_t wave = (blockIdx.x * kTB + threadIdx.x) >> 6;
+  const uint32_t nWaves = (gridDim.x * kTB) >> 6;
+  const uint32_t nx = (uint32_t)m.cx + 1u, ny = (uint32_t)m.cy + 1u, nz = (uint32_t)m.cz + 1u;
+  const uint32_t bx = (nx + 3u) / 4u, by = (ny + 1u) / 2u, bz = (nz + 1u) / 2u;
+  const uint64_t blocks = (uint64_t)bx * by * bz;
+  const uint32_t dx = (uint32_t)col & 3u, dy = ((uint32_t)col >> 2) & 1u, dz = (uint32_t)col >> 3;
+  const float* P = params + (size_t)net * kSdfStride;
+  const float* F = freq + (size_t)net * kSdfFreqs * 3;
+  const float* fr = frame + (size_t)net * 4;
+  bf16x8 frag[kSdfFrags];
+#pragma unroll
+  for (int f = 0; f < kSdfFrags; ++f) frag[f] = load_frag(packed, net, f, lane);
+  float over = -INFINITY, slope = 0.0f, err = 0.0f;
+  uint32_t count = 0;
+  for (uint64_t blk = wave; blk < blocks; blk += nWaves) {
+    const uint32_t kx = (uint32_t)(blk % bx), ky = (uint32_t)((blk / bx) % by),
+                   kz = (uint32_t)(blk / ((uint64_t)bx * by));
+    const uint32_t ix = 4u * kx + dx, iy = 2u * ky + dy, iz = 2u * kz + dz;
+    const bool valid = ix < nx && iy < ny && iz < nz;
+    float d = INFINITY, qx = 0.0f, qy = 0.0f, qz = 0.0f;
+    if (valid) {
+      d = m.cpd[((uint64_t)iz * ny + iy) * nx + ix].w;
+      qx = m.xs[ix];
+      qy = m.ys[iy];
+      qz = m.zs[iz];
+    }
+    const bool inBand = valid && d < radius + band;
+    if (__ballot(inBand) == 0ull) continue;
+    float x, y, z;
+    sdf_norm(fr, qx, qy, qz, x, y, z);
+    const float yv = sdf_tile_mfma(frag, P, F, x, y, z, g) * fr[3];
+    // neighbours along x (column + 1), y (+ 4) and z (+ 8) inside the block
+    const int base = lane & 48;
+    const int cX = (col + 1) & 15, cY = (col + 4) & 15, cZ = (col + 8) & 15;
+    const float yX = __shfl(yv, base | cX), yY = __shfl(yv, base | cY), yZ = __shfl(yv, base | cZ);
+    const float pX = __shfl(qx, base | cX), pY = __shfl(qy, base | cY), pZ = __shfl(qz, base | cZ);
+    const bool vX = __shfl((int)valid, base | cX) != 0, vY = __shfl((int)valid, base | cY) != 0,
+               vZ = __shfl((int)valid, base | cZ) != 0;
+    const bool bX = __shfl((int)inBand, base | cX) != 0, bY = __shfl((int)inBand, base | cY) != 0,
+               bZ = __shfl((int)inBand, base | cZ) != 0;
+    if (g == 0 && valid) {
+      if (inBand) {
+        over = fmaxf(over, yv - radius);
+        err = fmaxf(err, fabsf(yv - d));
+        ++count;
+      }
+      if (dx < 3u && vX && (inBand || bX)) slope = fmaxf(slope, fabsf(yX - yv) / (pX - qx));
+      if (dy == 0u && vY && (inBand || bY)) slope = fmaxf(slope, fabsf(yY - yv) / (pY - qy));
+      if (dz == 0u && vZ && (inBand || bZ)) slope = fmaxf(slope, fabsf(yZ - yv) / (pZ - qz));
+    }
+  }
+  // wave maxima, then one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    over = fmaxf(over, __shfl_xor(over, off));
+    slope = fmaxf(slope, __shfl_xor(slope, off));
+    err = fmaxf(err, __shfl_xor(err, off));
+    count += __shfl_xor(count, off);
+  }
+  if (lane == 0) {
+    atomicMax(&stats[0], float_key(over));
+    atomicMax(&stats[1], float_key(slope));
+    atomicAdd(&stats[2], count);
+    atomicMax(&stats[3], float_key(err));
+  }
+}
+
 // SplitMix64 stream for the initial weights and frequencies.
 struct Rng {
   uint64_t s;
@@ -1153,6 +1244,40 @@ fc_status fc_sdf_eval(fc_sdf* m, const uint32_t* bodies, const float* points, ui
     return set_error(FC_ERROR_INVALID_VALUE, "precision must be FC_SDF_FP32 or FC_SDF_BF16");
   }
   FC_HIP_TRY(hipGetLastError());
+  return FC_SUCCESS;
+}
+
+fc_status fc_sdf_map_margin(fc_sdf* m, uint32_t net, const fc_distmap* map, float radius,
+                            float band, float out[4]) {
+  if (!m || !map || !out) return set_error(FC_ERROR_INVALID_VALUE, "NULL argument");
+  if (net >= m->B) return set_error(FC_ERROR_INVALID_VALUE, "network index out of range");
+  if (!(band >= 0.0f)) return set_error(FC_ERROR_INVALID_VALUE, "band must be >= 0");
+  FC_TRY(sdf_pack(m));
+  hipStream_t st = m->ctx->stream;
+  uint32_t* d = nullptr;
+  FC_HIP_TRY(hipMalloc(&d, 4 * sizeof(uint32_t)));
+  const uint32_t init[4] = {0x007fffffu /* key of -inf */, 0u, 0u, 0u};
+  hipError_t e = hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sdf_map_margin, dim3(2048), dim3(kTB), 0, st, m->packed, m->params,
+                       m->freq, m->frame, net, distmap_view(map), radius, band, d);
+    e = hipGetLastError();
+  }
+  uint32_t h[4] = {0, 0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d);
+  if (e != hipSuccess) return hip_error(e, "sdf map margin");
+  auto unkey = [](uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  out[0] = unkey(h[0]);
+  out[1] = unkey(h[1]);
+  out[2] = (float)h[2];
+  out[3] = unkey(h[3]);
   return FC_SUCCESS;
 }
 

@@ -213,6 +213,36 @@ def sim_params_from_world(world: WorldParameters, n: int, grid: int, radius=None
     return P, float(world.time_step)
 
 
+@dataclass
+class PluginRun:
+    """What the FullC0ntact particle driver takes from a scene file
+    (Particles_GPUTest, applications/particles_gpu/particles_gpu.cpp:
+    44-218, 264-283): the number of pipeline steps, the VTK output cadence
+    and the rigid bodies whose distance maps bound the particles.  The
+    particle parameters are not in the scene: cuda_initParticles
+    (cuda_addon/particledem_um.cu:280-330) fixes them, and the scene's
+    gravity drives only the rigid-body world (particles_gpu.cpp:186)."""
+    params: "_lib.SimParams"
+    dt: float
+    steps: int
+    output_every: int
+    rigid_bodies: list
+
+
+def plugin_run_from_world(world: WorldParameters, n: int) -> PluginRun:
+    """The particle step of the CollisionPipeline<dem_gpu_unified> driver for
+    a scene: n spheres of radius 1/64 on a 64^3 grid from (-1, -1, -1),
+    spring .5, damping .02, shear .1, no attraction, gravity -0.0003 in z,
+    boundary damping -.5, global damping 1 and dt .5 (particledem_um.cu:
+    285-297); the run loop steps while the time-step counter is at most
+    totalTimesteps (particles_gpu.cpp:266) and writes the particle VTK dump
+    every 250 steps (:278-279).  The plug-in's evalForces has no cursor
+    sphere, so it is moved out of reach."""
+    from .particle_system import make_params
+    P = make_params(n, 64, 1.0 / 64.0, colliderPos=(1.0e4, 1.0e4, 1.0e4), colliderRadius=0.0)
+    return PluginRun(P, 0.5, int(world.total_timesteps), 250, list(world.rigid_bodies))
+
+
 def write_particle_vtk(path, pos4, radius: float = 1.0 / 64.0, radius_from_w: bool = False):
     """WriteGPUParticleFile for host float4 positions (n,4).  The reference
     writes radius 1/64 for every sphere (the default here)."""

@@ -67,6 +67,17 @@ class SdfSurrogate:
                                hist.ctypes.data))
         return hist
 
+    def map_margin(self, net: int, distance_map, radius: float, band: float) -> dict:
+        """Statistics of network `net` (BF16 inference) against a distance
+        map of its shape over the lattice vertices within radius + band of
+        the surface: the largest y - radius, the largest secant slope
+        between x-neighbours, the vertex count and the largest |y - d|."""
+        out = np.zeros(4, np.float32)
+        check(lib.fc_sdf_map_margin(self.handle, int(net), distance_map.handle,
+                                    C.c_float(radius), C.c_float(band), out.ctypes.data))
+        return {"over": float(out[0]), "slope": float(out[1]), "vertices": int(out[2]),
+                "max_err": float(out[3])}
+
     def eval(self, points, bodies=None, precision: int = BF16) -> np.ndarray:
         """Signed distances of host model-frame points (n, 3)."""
         pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)

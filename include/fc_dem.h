@@ -174,6 +174,19 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
                      const uint32_t* gridParticleIndex, const uint32_t* cellStart,
                      const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells);
 
+/* evalForces (particledem_um.cuh:19; evalForces_launcher,
+ * particledem_um.cu:538-589), the force stage of the FullC0ntact plug-in
+ * pipeline: the same 27-cell sum as fc_collide but without the cursor
+ * sphere, written as newVel[index[k]] = (vel + force, 0) and
+ * force[3 * index[k] + 0..2] = force (float3 array, 12-byte stride, the
+ * ParticleWorld::force_ layout).  The reference reads the grid and the
+ * particle arrays from its HashGrid/ParticleWorld structs; here they are
+ * passed as device pointers, with the parameters of the context. */
+fc_status fc_eval_forces(fc_context* ctx, float* newVel, float* force, const float* sortedPos,
+                         const float* sortedVel, const uint32_t* gridParticleIndex,
+                         const uint32_t* cellStart, const uint32_t* cellEnd,
+                         uint32_t numParticles, uint32_t numCells);
+
 /* ------------------------------------------------------------------------- */
 /* Particle system (ParticleSystem class, particleSystem.h:22-136).          */
 /* The system owns every device buffer; update() runs the reference order     */
@@ -464,6 +477,15 @@ fc_status fc_sdf_loss_grad(fc_sdf* sdf, const float* points, const float* labels
 fc_status fc_sdf_train(fc_sdf* sdf, const float* points, const float* labels,
                        uint32_t samplesPerBody, uint32_t batchPerBody, uint32_t epochs, float lr,
                        float lrDecay, float* lossHistory);
+/* Calibration statistics of network `net` against a distance map of its
+ * shape (model frame), over the map's lattice vertices v with
+ * d(v) < radius + band, the network evaluated by the BF16 MFMA inference of
+ * the cull: out[0] = max (y(v) - radius), out[1] = max secant slope
+ * |y(v') - y(v)| / |v' - v| between lattice neighbours along x, y and z,
+ * out[2] = the number of such vertices, out[3] = max |y(v) - d(v)|.  No
+ * reference counterpart (the surrogate is an addition). */
+fc_status fc_sdf_map_margin(fc_sdf* sdf, uint32_t net, const fc_distmap* map, float radius,
+                            float band, float out[4]);
 /* Signed distance of n model-frame points (device, 3 floats each) of the
  * given bodies (device u32 per point; NULL = body 0) into out (device);
  * precision FC_SDF_FP32 or FC_SDF_BF16.  A body id >= numBodies gives NaN. */
@@ -574,6 +596,14 @@ fc_status fc_context_get_timing(fc_context* ctx, fc_stage_timing* out, int max, 
  * profiler and measures the achievable copy bandwidth of the device. */
 fc_status fc_copy_probe(fc_context* ctx, void* dst, const void* src, size_t bytes, int iterations,
                         float* avgMs);
+/* No counterpart in the reference: the cost of separating the stages of a
+ * step by a grid-wide barrier inside one persistent (cooperative) launch of
+ * `blocks` blocks of 256 threads, against a kernel boundary inside a
+ * captured graph of `iterations` such kernels; out[0] = microseconds per
+ * grid barrier, out[1] = microseconds per graph kernel.  Measures whether a
+ * single persistent launch per batched step could beat the graph-launched
+ * stages. */
+fc_status fc_sync_probe(fc_context* ctx, uint32_t blocks, int iterations, float out[2]);
 /* Lane-layout check of the 16x16x32 BF16 MFMA as the surrogate kernels use
  * it: d = a b for a 16x32 and b 32x16 row-major bf16 matrices, d 16x16
  * row-major f32 (device arrays).  No counterpart in the reference. */

@@ -226,10 +226,13 @@ void orc_pair_force(const float* pA, const float* pB, const float* vA, const flo
 }
 
 /* collideD + collideCell (particles_kernel.cu:223-293); with poly each
- * pair uses the spheres' own radii pos.w (the collider pair uses rA). */
+ * pair uses the spheres' own radii pos.w (the collider pair uses rA).
+ * collider == 0 with force3 != NULL is evalForces_launcher
+ * (particledem_um.cu:538-589): the same 27-cell sum without the cursor
+ * sphere, and the force is also stored as float3 at the original index. */
 static void collide_impl(float* newVel4, const float* sortedPos4, const float* sortedVel4,
                          const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
-                         uint32_t n, const orc_params* P, int poly) {
+                         uint32_t n, const orc_params* P, int poly, int collider, float* force3) {
   long k;
 #pragma omp parallel for schedule(dynamic, 1024)
   for (k = 0; k < (long)n; ++k) {
@@ -261,11 +264,18 @@ static void collide_impl(float* newVel4, const float* sortedPos4, const float* s
           }
           force = f3_add(force, cellForce);
         }
-    force = f3_add(force, pair_force(pos, f3_make(P->colliderPos.x, P->colliderPos.y,
-                                                  P->colliderPos.z),
-                                     vel, f3_make(0.0f, 0.0f, 0.0f), rA, P->colliderRadius, 0.0f,
-                                     P));
+    if (collider)
+      force = f3_add(force, pair_force(pos, f3_make(P->colliderPos.x, P->colliderPos.y,
+                                                    P->colliderPos.z),
+                                       vel, f3_make(0.0f, 0.0f, 0.0f), rA, P->colliderRadius,
+                                       0.0f, P));
     uint32_t orig = index[k];
+    if (force3) {
+      float* fo = force3 + 3 * (size_t)orig;
+      fo[0] = force.x;
+      fo[1] = force.y;
+      fo[2] = force.z;
+    }
     float* out = newVel4 + 4 * (size_t)orig;
     out[0] = vel.x + force.x;
     out[1] = vel.y + force.y;
@@ -277,13 +287,19 @@ static void collide_impl(float* newVel4, const float* sortedPos4, const float* s
 void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel4,
                  const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
                  uint32_t n, const orc_params* P) {
-  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 0);
+  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 0, 1, NULL);
+}
+
+void orc_eval_forces(float* newVel4, float* force3, const float* sortedPos4,
+                     const float* sortedVel4, const uint32_t* index, const uint32_t* cellStart,
+                     const uint32_t* cellEnd, uint32_t n, const orc_params* P) {
+  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 0, 0, force3);
 }
 
 void orc_collide_poly(float* newVel4, const float* sortedPos4, const float* sortedVel4,
                       const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
                       uint32_t n, const orc_params* P) {
-  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 1);
+  collide_impl(newVel4, sortedPos4, sortedVel4, index, cellStart, cellEnd, n, P, 1, 1, NULL);
 }
 
 /* Contacts per sorted sphere: neighbours j != k in the 27 cells with
@@ -415,7 +431,7 @@ void orc_update_ex(orc_state* s, float dt, const orc_params* P, const orc_distma
   orc_reorder(s->cellStart, s->cellEnd, s->sortedPos4, s->sortedVel4, s->hash, s->index,
               s->pos4, s->vel4, s->n, P->numCells);
   collide_impl(s->vel4, s->sortedPos4, s->sortedVel4, s->index, s->cellStart, s->cellEnd, s->n, P,
-               poly);
+               poly, 1, NULL);
 }
 
 /* SplitMix64 (Steele, Lea, Flood 2014). */

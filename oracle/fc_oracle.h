@@ -69,6 +69,12 @@ void orc_collide(float* newVel4, const float* sortedPos4, const float* sortedVel
                  uint32_t n, const orc_params* P);
 /* Same algorithm in binary64 from the same binary32 inputs: newVel3 receives
  * (vel + force) per original index as doubles (3 per particle). */
+/* evalForces_launcher (particledem_um.cu:538-589) in binary32: collideD's
+ * 27-cell sum without the cursor sphere; newVel[orig] = (vel + force, 0)
+ * and force3[orig] = force (float3, 12-byte stride). */
+void orc_eval_forces(float* newVel4, float* force3, const float* sortedPos4,
+                     const float* sortedVel4, const uint32_t* index, const uint32_t* cellStart,
+                     const uint32_t* cellEnd, uint32_t n, const orc_params* P);
 void orc_collide_f64(double* newVel3, const float* sortedPos4, const float* sortedVel4,
                      const uint32_t* index, const uint32_t* cellStart, const uint32_t* cellEnd,
                      uint32_t n, const orc_params* P);

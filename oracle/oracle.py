@@ -86,6 +86,7 @@ def _load() -> C.CDLL:
         "orc_sort_pairs": (None, [vp, vp, u32]),
         "orc_reorder": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32]),
         "orc_collide": (None, [vp, vp, vp, vp, vp, vp, u32, P]),
+        "orc_eval_forces": (None, [vp, vp, vp, vp, vp, vp, vp, u32, P]),
         "orc_collide_f64": (None, [vp, vp, vp, vp, vp, vp, u32, P]),
         "orc_pair_force": (None, [vp, vp, vp, vp, f, f, f, P, vp]),
         "orc_update": (None, [C.POINTER(OrcState), f, P]),
@@ -207,6 +208,21 @@ def collide(sp, sv, index, cs, ce, P, newVel4=None):
     lib.orc_collide(_p(out), _p(sp), _p(sv), _p(index), _p(np.ascontiguousarray(cs)),
                     _p(np.ascontiguousarray(ce)), n, C.byref(P))
     return out
+
+
+def eval_forces(sp, sv, index, cs, ce, P, newVel4=None):
+    """evalForces_launcher (particledem_um.cu:538-589): (newVel4, force3),
+    the 27-cell force without the cursor sphere, both at the original
+    indices."""
+    n = len(sp)
+    out = np.zeros((n, 4), np.float32) if newVel4 is None else np.array(newVel4, np.float32)
+    force = np.zeros((n, 3), np.float32)
+    sp = np.ascontiguousarray(sp, np.float32)
+    sv = np.ascontiguousarray(sv, np.float32)
+    index = np.ascontiguousarray(index, np.uint32)
+    lib.orc_eval_forces(_p(out), _p(force), _p(sp), _p(sv), _p(index),
+                        _p(np.ascontiguousarray(cs)), _p(np.ascontiguousarray(ce)), n, C.byref(P))
+    return out, force
 
 
 def integrate_poly(pos4, vel4, dt, P):

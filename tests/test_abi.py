@@ -26,6 +26,7 @@ def test_header_declares_the_stage_entry_points():
     names = declared_functions()
     for required in ("fc_set_parameters", "fc_integrate_system", "fc_calc_hash",
                      "fc_sort_particles", "fc_reorder_data_and_find_cell_start", "fc_collide",
+                     "fc_eval_forces",
                      "fc_system_create", "fc_system_update", "fc_system_run",
                      "fc_system_get_array", "fc_system_set_array"):
         assert required in names

@@ -99,6 +99,17 @@ class Stages:
         self.ctx.sync()
         return dnv.download(np.float32, 4 * n).reshape(n, 4)
 
+    def eval_forces(self, sp, sv, idx, cs, ce, newvel_init):
+        n, C = len(sp), self.P.numCells
+        dnv = self.dev(newvel_init)
+        dforce = self.dev(np.full((n, 3), np.nan, np.float32))
+        d = [self.dev(x) for x in (sp, sv, idx, cs, ce)]
+        self.L.check(self.L.lib.fc_eval_forces(self.ctx.handle, dnv.ptr, dforce.ptr, d[0].ptr,
+                                               d[1].ptr, d[2].ptr, d[3].ptr, d[4].ptr, n, C))
+        self.ctx.sync()
+        return (dnv.download(np.float32, 4 * n).reshape(n, 4),
+                dforce.download(np.float32, 3 * n).reshape(n, 3))
+
     def integrate(self, pos, vel, dt):
         n = len(pos)
         dp, dv = self.dev(pos), self.dev(vel)
@@ -155,6 +166,48 @@ def test_stages_bitexact(gpu_ctx, case):
     err = np.max(np.abs(force32 - force64)) / scale
     print(f"[stages] case={case} max |F32-F64| / max|F64| = {err:.3e}", flush=True)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("case", ["cfg0-collider-in-bed", "ragged-128", "grid-2x64x64"])
+def test_eval_forces_bitexact(gpu_ctx, case):
+    """fc_eval_forces (evalForces_launcher, particledem_um.cu:538-589): the
+    27-cell force without the cursor sphere, newVel and force3 at the
+    original indices, bit for bit against the oracle's twin.  The cursor
+    sphere is placed inside the bed, so fc_collide and fc_eval_forces must
+    differ exactly on the spheres it touches."""
+    if case == "cfg0-collider-in-bed":
+        n, r = 4096, 1.0 / 64.0
+        P = sim_params(n, 64, r, colliderPos=(-0.75, -0.75, -0.75), colliderRadius=0.1)
+    elif case == "ragged-128":
+        n, r = 100_003, 1.0 / 128.0
+        P = sim_params(n, 128, r, colliderPos=(-0.6, -0.6, -0.6), colliderRadius=0.05,
+                       attraction=0.001)
+    else:  # the per-cell loop kernel (a grid dimension below 4)
+        n, r = 3000, 1.0 / 64.0
+        P = sim_params(n, 64, r, gridSize=(2, 64, 64), numCells=2 * 64 * 64,
+                       colliderPos=(-0.8, -0.8, -0.8), colliderRadius=0.1)
+    print(f"[eval_forces] case={case} n={n}", flush=True)
+    OP = orc.params_from(P)
+    S = Stages(gpu_ctx, P)
+    pos, vel = lattice(n, r, 64)
+    op1, ov1 = orc.integrate(pos, vel, 0.5, OP)
+    oh, oi = orc.calc_hash(op1, OP)
+    osh, osi = orc.sort_pairs(oh, oi)
+    ocs, oce, osp, osv = orc.reorder(osh, osi, op1, ov1, P.numCells)
+    nv, force = S.eval_forces(osp, osv, osi, ocs, oce, ov1)
+    onv, oforce = orc.eval_forces(osp, osv, osi, ocs, oce, OP, newVel4=ov1)
+    assert np.array_equal(bits(nv), bits(onv))
+    assert np.array_equal(bits(force), bits(oforce))
+    assert np.count_nonzero(np.any(oforce != 0, axis=1)) > n // 10  # contacts exist
+    # the same cell sum as collide, without the cursor sphere
+    onv_c = orc.collide(osp, osv, osi, ocs, oce, OP, newVel4=ov1)
+    differ = np.any(bits(onv_c) != bits(onv), axis=1)
+    dist = np.linalg.norm(op1[:, :3].astype(np.float64) -
+                          np.array([P.colliderPos.x, P.colliderPos.y, P.colliderPos.z]), axis=1)
+    reach = P.colliderRadius + r
+    assert differ.sum() > 100
+    assert np.all(dist[differ] < reach * (1 + 1e-6))  # only spheres the cursor sphere touches
+    assert not np.any(differ[dist < reach * (1 - 1e-6)] == False)  # noqa: E712
 
 
 @pytest.mark.parametrize("n", [1, 2, 255, 4095, 4096, 4097, 100_003, 1 << 20])
@@ -260,14 +313,38 @@ def test_run_graph_equals_repeated_update(gpu_ctx):
     b.close()
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4095, 4096, 4097])
-def test_small_systems_run_and_update_match_oracle(gpu_ctx, n):
+SMALL_GRIDS = {
+    # the single-block sort takes 9-bit digits over the width of numCells - 1:
+    # 8^3 = 9 bits (one pass), 32^3 = 15 bits (two), 64^3 = 18 bits (two),
+    # 128^3 = 21 bits (three), 64 x 64 x 16 = 16 bits (two, non-cubic).
+    # The cell stays 2r = 1/32, so the smaller grids wrap: many spheres of
+    # distant cells share a hash cell.
+    "8^3": (8, 8, 8),
+    "32^3": (32, 32, 32),
+    "64^3": (64, 64, 64),
+    "128^3": (128, 128, 128),
+    "64x64x16": (64, 64, 16),
+}
+
+
+SMALL_CASES = [(n, "64^3") for n in (1, 2, 63, 65, 1000, 4095, 4096, 4097)] + [
+    (n, g) for g in SMALL_GRIDS if g != "64^3" for n in (1, 65, 4095, 4096, 4097)]
+
+
+@pytest.mark.parametrize("n,grid_name", SMALL_CASES)
+def test_small_systems_run_and_update_match_oracle(gpu_ctx, n, grid_name):
     # up to one sort tile (4096) a step sorts in one block that also writes
     # the cell tables; 4097 takes the multi-block sort
-    grid, r = 64, 1.0 / 64.0
-    print(f"[small] n={n}", flush=True)
+    from fullc0ntact_amd import ParticleSystem
+    gs, r = SMALL_GRIDS[grid_name], 1.0 / 64.0
+    print(f"[small] n={n} grid={grid_name}", flush=True)
     for mode in ("run", "update"):
-        ps, P, pos, vel = _system(n, grid, r, seed=5)
+        P = sim_params(n, gs[0], r, gridSize=gs, numCells=gs[0] * gs[1] * gs[2])
+        ps = ParticleSystem(n, gs)
+        ps.m_params = P
+        pos, vel = lattice(n, r, gs[0], 5, 0.1, 0.01)
+        ps.setArray(ps.POSITION, pos, 0, n)
+        ps.setArray(ps.VELOCITY, vel, 0, n)
         st = orc.State(pos, vel, P.numCells)
         OP = orc.params_from(P)
         steps = 12 if mode == "run" else 3
@@ -491,11 +568,14 @@ def test_collide_list_overflow_path(gpu_ctx):
     assert np.array_equal(bits(nv), bits(onv))
 
 
-def test_system_list_overflow_fallback(gpu_ctx):
+@pytest.mark.parametrize("poly", [False, True], ids=["mono", "poly"])
+def test_system_list_overflow_fallback(gpu_ctx, poly):
     # Clusters of 30-60 heavily overlapping spheres among loose ones: the
     # clustered lanes hold more contacts than the contact list and take the
     # row-range fallback, beside list-path lanes in the same waves.  Updates
-    # and a graph-launched run must equal the oracle bit for bit.
+    # and a graph-launched run must equal the oracle bit for bit.  With poly
+    # the radii come from pos.w (U(0.6 r, r)), through the fallback's
+    # polydisperse branch.
     from fullc0ntact_amd import ParticleSystem
     grid, r = 64, 1.0 / 64.0
     rng = np.random.default_rng(44)
@@ -508,35 +588,43 @@ def test_system_list_overflow_fallback(gpu_ctx):
     n = len(xyz)
     pos = np.zeros((n, 4), np.float32)
     pos[:, :3] = np.clip(xyz, -1 + r, 1 - r)
+    pos[:, 3] = rng.uniform(0.6 * r, r, n).astype(np.float32) if poly else 1.0
     vel = np.zeros((n, 4), np.float32)
     vel[:, :3] = rng.uniform(-0.01, 0.01, size=(n, 3))
     P = sim_params(n, grid, r)
     OP = orc.params_from(P)
+
+    def step(st):
+        orc.update_ex(st, 0.5, OP, None, poly=poly)
+
     ps = ParticleSystem(n, (grid, grid, grid), context=gpu_ctx)
     ps.m_params = P
+    ps.setPolydisperse(poly)
     ps.setArray(ps.POSITION, pos, 0, n)
     ps.setArray(ps.VELOCITY, vel, 0, n)
     st = orc.State(pos, vel, P.numCells)
-    print(f"[overflow] n={n}", flush=True)
+    print(f"[overflow] n={n} poly={poly}", flush=True)
     ps.update(0.5)
-    st.update(0.5, OP)
+    step(st)
     _compare_system(ps, st)
     # the first update's collide saw the clusters whole
-    contacts = np.asarray(orc.contact_counts(st.sortedPos4, st.cellStart, st.cellEnd, OP))
+    contacts = np.asarray(orc.contact_counts(st.sortedPos4, st.cellStart, st.cellEnd, OP,
+                                             poly=poly))
     assert contacts.max() > 40 and np.count_nonzero(contacts > 28) > 500
     ps.update(0.5)
-    st.update(0.5, OP)
+    step(st)
     _compare_system(ps, st)
     ps.close()
     # the fused, graph-launched steps of a run from the same clusters
     ps = ParticleSystem(n, (grid, grid, grid), context=gpu_ctx)
     ps.m_params = P
+    ps.setPolydisperse(poly)
     ps.setArray(ps.POSITION, pos, 0, n)
     ps.setArray(ps.VELOCITY, vel, 0, n)
     st = orc.State(pos, vel, P.numCells)
     ps.run(0.5, 11)
     for _ in range(11):
-        st.update(0.5, OP)
+        step(st)
     assert np.array_equal(bits(ps.getArray(ps.POSITION)), bits(st.pos4))
     assert np.array_equal(bits(ps.getArray(ps.VELOCITY)), bits(st.vel4))
     ps.close()
