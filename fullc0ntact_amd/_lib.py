@@ -131,6 +131,8 @@ _SIGS = {
     "fc_collide": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32]),
     "fc_eval_forces": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32]),
     "fc_sync_probe": (_st, [_vp, _u32, C.c_int, _vp]),
+    "fc_sort_particles_noalloc": (_st, [_vp, _vp, _vp, _u32]),
+    "fc_sort_particles_noalloc_max": (C.c_size_t, []),
     "fc_system_create": (_st, [_vp, _u32, UInt3, C.POINTER(_vp)]),
     "fc_system_destroy": (_st, [_vp]),
     "fc_system_num_particles": (_u32, [_vp]),

@@ -232,6 +232,13 @@ fc_status fc_context_create(int device, fc_context** out) {
   if (device >= 0) FC_HIP_TRY(hipSetDevice(device));
   fc_context* ctx = new fc_context();
   FC_HIP_TRY(hipGetDevice(&ctx->device));
+  {
+    const fc_status st = init_noalloc_scratch(ctx->device);
+    if (st != FC_SUCCESS) {
+      delete ctx;
+      return st;
+    }
+  }
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;

@@ -267,6 +267,9 @@ fc_status ensure_sort_capacity(fc_context* ctx, size_t n);
 // Synchronises the context stream and converts the device error word.
 fc_status check_device_flags(fc_context* ctx);
 fc_sim_params default_params(uint32_t numParticles, fc_uint3 gridSize);
+// Resolves, once per device, the device address of the no-allocation sort's
+// scratch array (sort_static.hip); called with `device` current.
+fc_status init_noalloc_scratch(int device);
 
 }  // namespace fc
 

@@ -156,6 +156,19 @@ fc_status fc_calc_hash(fc_context* ctx, uint32_t* gridParticleHash, uint32_t* gr
 fc_status fc_sort_particles(fc_context* ctx, uint32_t* gridParticleHash, uint32_t* gridParticleIndex,
                             uint32_t numParticles);
 
+/* No counterpart in the reference: the sort of fc_sort_particles for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * allocating routine: its scratch is one module array per device (about
+ * 16.6 MiB, held until the process ends), and it sorts at most
+ * fc_sort_particles_noalloc_max() = 1048576 pairs (FC_ERROR_CAPACITY
+ * beyond).  Every call on a device shares that device's array: calls that
+ * overlap on one device (other streams or threads) race; calls on one
+ * stream are ordered.  The results equal fc_sort_particles'. */
+fc_status fc_sort_particles_noalloc(fc_context* ctx, uint32_t* gridParticleHash,
+                                    uint32_t* gridParticleIndex, uint32_t numParticles);
+size_t fc_sort_particles_noalloc_max(void);
+
 /* reorderDataAndFindCellStart (particleSystem.cuh:41-50; kernel
  * particles_kernel.cu:119-182).  Clears cellStart AND cellEnd to
  * FC_CELL_EMPTY first (as particledem.cu:229-230 does) so every table entry
