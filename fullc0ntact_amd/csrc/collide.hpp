@@ -489,4 +489,73 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
   return force;
 }
 
+// One neighbour row of collide_one_rows for the row-split collide of small
+// systems, where nine lanes share a sphere (row r = 3 (dz + 1) + (dy + 1)):
+// the partial force sums of the row's three cells (dx = -1, 0, 1), each in
+// index order from +0, with every candidate through the exact pair force
+// (non-touching ones add +0).  Cells out of contact reach are cut as in
+// collide_one_rows (their partials stay +0, which is what the reference
+// adds for them); the cells are walked one by one, so a row that wraps in x
+// needs no special case.  Requires every grid dimension >= 4.
+template <bool kPoly>
+__device__ __forceinline__ void collide_row_cells(const fc_sim_params& P, v3f invCell,
+                                                  float contactD2, uint32_t k, v3f pos, v3f vel,
+                                                  float rA, int r,
+                                                  const float4* __restrict__ sortedPos,
+                                                  const float4* __restrict__ sortedVel,
+                                                  const uint32_t* __restrict__ cellStart,
+                                                  const uint32_t* __restrict__ cellEnd,
+                                                  uint32_t n, v3f (&cf)[3]) {
+  const v3f q = cell_quotients(P, invCell, pos);
+  const int gx = cell_of_quotient(q.x), gy = cell_of_quotient(q.y), gz = cell_of_quotient(q.z);
+  const uint32_t Gx = P.gridSize.x, Gy = P.gridSize.y;
+  const uint32_t mx = Gx - 1u, my = Gy - 1u, mz = P.gridSize.z - 1u;
+  const int dy = r % 3 - 1, dz = r / 3 - 1;
+  const uint32_t yy = ((uint32_t)gy + (uint32_t)dy) & my;
+  const uint32_t zz = ((uint32_t)gz + (uint32_t)dz) & mz;
+  const uint32_t rowBase = (zz * Gy + yy) * Gx;
+  uint32_t st[3], en[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t xx = ((uint32_t)gx + (uint32_t)(c - 1)) & mx;
+    st[c] = cellStart[rowBase + xx];
+    en[c] = cellEnd[rowBase + xx];
+  }
+  // lower bounds on the squared distance to the row's cells (collide_one_rows)
+  const float thrA = fmaxf(contactD2 * (1.0f + 0x1p-18f), contactD2 + 0x1p-140f);
+  const float thrCut = thrA * (1.0f + 0x1p-16f);
+  const float fx = q.x - floorf(q.x), fy = q.y - floorf(q.y), fz = q.z - floorf(q.z);
+  const float ux = fmaf(fabsf(q.x), 0x1p-20f, 0x1p-20f);
+  const float uy = fmaf(fabsf(q.y), 0x1p-20f, 0x1p-20f);
+  const float uz = fmaf(fabsf(q.z), 0x1p-20f, 0x1p-20f);
+  const float lx = fmaxf(fx - ux, 0.0f) * P.cellSize.x, hx = fmaxf(1.0f - fx - ux, 0.0f) * P.cellSize.x;
+  const float ly = fmaxf(fy - uy, 0.0f) * P.cellSize.y, hy = fmaxf(1.0f - fy - uy, 0.0f) * P.cellSize.y;
+  const float lz = fmaxf(fz - uz, 0.0f) * P.cellSize.z, hz = fmaxf(1.0f - fz - uz, 0.0f) * P.cellSize.z;
+  const float gyz = (dy < 0 ? ly * ly : (dy > 0 ? hy * hy : 0.0f)) +
+                    (dz < 0 ? lz * lz : (dz > 0 ? hz * hz : 0.0f));
+  const bool keep[3] = {!(gyz + lx * lx >= thrCut), !(gyz >= thrCut), !(gyz + hx * hx >= thrCut)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cf[c] = make_v3(0.0f, 0.0f, 0.0f);
+    if (st[c] == CELL_EMPTY || !keep[c]) continue;
+    const uint32_t e = en[c] > n ? n : en[c];
+#pragma unroll 1
+    for (uint32_t j0 = st[c]; j0 < e; j0 += 2) {
+      const uint32_t j1 = j0 + 1u < e ? j0 + 1u : j0;
+      const float4 p0 = sortedPos[j0], p1 = sortedPos[j1];
+      const float4 v0 = sortedVel[j0], v1 = sortedVel[j1];
+      if (j0 != k) {
+        const v3f f = collide_spheres_fast(pos, xyz(p0), vel, xyz(v0), rA, kPoly ? p0.w : rA,
+                                           P.spring, P.damping, P.shear, P.attraction);
+        cf[c] = make_v3(cf[c].x + f.x, cf[c].y + f.y, cf[c].z + f.z);
+      }
+      if (j0 + 1u < e && j1 != k) {
+        const v3f f = collide_spheres_fast(pos, xyz(p1), vel, xyz(v1), rA, kPoly ? p1.w : rA,
+                                           P.spring, P.damping, P.shear, P.attraction);
+        cf[c] = make_v3(cf[c].x + f.x, cf[c].y + f.y, cf[c].z + f.z);
+      }
+    }
+  }
+}
+
 }  // namespace fc

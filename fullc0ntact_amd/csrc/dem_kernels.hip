@@ -334,6 +334,98 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
   }
 }
 
+// The collide of small systems (few spheres: the per-lane walk of
+// k_collide_v3 is then bound by the latency of its chain of dependent
+// loads, not by throughput), with each sphere's nine neighbour rows on nine
+// lanes of a wave (collide_row_cells: seven spheres per wave, lane 63
+// idle).  The row-0 lane gathers the other rows' cell partials by lane
+// shuffles and adds the 27 in the reference order (z, then y, then x), then
+// the cursor sphere; the rest is k_collide_v3's tail (the scatter, or with
+// kFuse the next update's integrate, hash and histogram), with the same
+// arguments and results.
+constexpr int kSplitSpheresPerWave = 7;
+
+template <bool kFuse, bool kPoly>
+__global__ __launch_bounds__(kBlock) void k_collide_split(
+    fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
+    const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
+    const uint32_t* __restrict__ index, const uint32_t* __restrict__ cellStart,
+    const uint32_t* __restrict__ cellEnd, uint32_t n, float4* __restrict__ pos, float dt,
+    uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
+    uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
+    v3f invCell, float* __restrict__ forceOut) {
+  __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int sl = lane / 9, r = lane - 9 * (lane / 9);
+  const uint32_t k = wave * kSplitSpheresPerWave + (uint32_t)sl;
+  const bool valid = lane < 9 * kSplitSpheresPerWave && k < n;
+  if (kFuse) {
+    for (int i = threadIdx.x; i < RS_RADIX; i += kBlock) s_hist[i] = 0u;
+    histogram_producer_prologue(gHist, passes, counters);
+  }
+  v3f cf[3];
+  float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  v3f vel0 = make_v3(0.0f, 0.0f, 0.0f);
+  float rA = P.particleRadius;
+  if (valid) {
+    p4 = sortedPos[k];
+    vel0 = xyz(sortedVel[k]);
+    rA = kPoly ? p4.w : P.particleRadius;
+    collide_row_cells<kPoly>(P, invCell, contactD2, k, xyz(p4), vel0, rA, r, sortedPos,
+                             sortedVel, cellStart, cellEnd, n, cf);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cf[c] = make_v3(0.0f, 0.0f, 0.0f);
+  }
+  // the row-0 lane of each sphere adds the 27 cell partials in order
+  const int base = 9 * sl;
+  v3f force = make_v3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int rr = 0; rr < 9; ++rr) {
+    const int src = base + rr < 64 ? base + rr : 63;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float x = __shfl(cf[c].x, src), y = __shfl(cf[c].y, src), z = __shfl(cf[c].z, src);
+      force = make_v3(force.x + x, force.y + y, force.z + z);
+    }
+  }
+  uint32_t key = 0;
+  const bool owner = valid && r == 0;
+  if (owner) {
+    if (kPoly && !(rA <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
+    const v3f pos0 = xyz(p4);
+    const uint32_t orig = index[k];
+    if (kFuse || forceOut == nullptr) {
+      force = add_collider(P, pos0, vel0, rA, force);
+    } else {  // evalForces: no cursor sphere, the force stored as float3
+      forceOut[3 * (size_t)orig] = force.x;
+      forceOut[3 * (size_t)orig + 1] = force.y;
+      forceOut[3 * (size_t)orig + 2] = force.z;
+    }
+    const v3f nv = make_v3(vel0.x + force.x, vel0.y + force.y, vel0.z + force.z);
+    if (!kFuse) {
+      newVel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+      if (pos) pos[orig] = p4;
+    } else {
+      v3f p = pos0, v = nv;
+      integrate_with(P, B, p4.w, dt, p, v);
+      pos[k] = make_float4(p.x, p.y, p.z, p4.w);
+      newVel[k] = make_float4(v.x, v.y, v.z, 0.0f);
+      key = hash_of_q(P, cell_quotients(P, invCell, p));
+      keysOut[orig] = key;
+      slotOf[orig] = k;
+      idOfSlot[k] = orig;
+    }
+  }
+  if (kFuse) {
+    __syncthreads();
+    histogram_add_digit(s_hist, key & (RS_RADIX - 1), owner);
+    __syncthreads();
+    histogram_flush(s_hist, gHist, 0);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count) {
@@ -456,6 +548,12 @@ uint32_t quant_threshold(float R, float scale) {
   return (uint32_t)std::ceil(b * b);
 }
 
+// Systems up to this many spheres take the row-split collide (k_collide_split).
+#ifndef FC_SPLIT_MAX_SPHERES
+#define FC_SPLIT_MAX_SPHERES 16384
+#endif
+constexpr uint32_t kSplitMaxSpheres = FC_SPLIT_MAX_SPHERES;
+
 template <bool kFuse, bool kPoly>
 static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                       const fc_sim_params& P,
@@ -465,7 +563,13 @@ static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
                       uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
                       float* forceOut = nullptr) {
-  if (gs != nullptr && rows_walk_ok(P))
+  if (gs != nullptr && rows_walk_ok(P) && n <= kSplitMaxSpheres) {
+    const uint32_t waves = (n + kSplitSpheresPerWave - 1) / kSplitSpheresPerWave;
+    hipLaunchKernelGGL((k_collide_split<kFuse, kPoly>), dim3((waves + 3) / 4), b, 0, s, P, B, d2,
+                       newVel, sortedPos, sortedVel, index, cellStart, cellEnd, n, pos, dt,
+                       keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot, inv_cell(P),
+                       forceOut);
+  } else if (gs != nullptr && rows_walk_ok(P))
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, true>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,

@@ -173,6 +173,12 @@ fc_status ensure_sort_capacity(fc_context* ctx, size_t n) {
   return FC_SUCCESS;
 }
 
+// The context's stage timer when timing is enabled (fc_context_enable_timing),
+// so that the per-stage entry points can be timed like a system's stages.
+static StageTimer* stage_timer(fc_context* ctx) {
+  return ctx->timer.enabled ? &ctx->timer : nullptr;
+}
+
 static fc_status sort_pairs(fc_context* ctx, const uint32_t* keysIn, const uint32_t* valsIn,
                             uint32_t* keysOut, uint32_t* valsOut, uint32_t n, uint32_t numCells,
                             StageTimer* timer) {
@@ -344,8 +350,11 @@ fc_status fc_integrate_system(fc_context* ctx, float* pos, float* vel, float del
   if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
   if (numParticles == 0) return FC_SUCCESS;
   if (!pos || !vel) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  StageTimer* t = stage_timer(ctx);
+  if (t) t->begin(ctx->stream, "drop_in_integrate");
   hipError_t e = launch_integrate(ctx->stream, ctx->params, (float4*)pos, (float4*)vel, deltaTime,
                                   numParticles);
+  if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "integrate");
   return FC_SUCCESS;
 }
@@ -355,8 +364,11 @@ fc_status fc_calc_hash(fc_context* ctx, uint32_t* hash, uint32_t* index, const f
   if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
   if (numParticles == 0) return FC_SUCCESS;
   if (!hash || !index || !pos) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  StageTimer* t = stage_timer(ctx);
+  if (t) t->begin(ctx->stream, "drop_in_calc_hash");
   hipError_t e = launch_calc_hash(ctx->stream, ctx->params, hash, index, (const float4*)pos,
                                   numParticles);
+  if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "calcHash");
   return FC_SUCCESS;
 }
@@ -366,7 +378,8 @@ fc_status fc_sort_particles(fc_context* ctx, uint32_t* hash, uint32_t* index,
   if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
   if (numParticles == 0) return FC_SUCCESS;
   if (!hash || !index) return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
-  return sort_pairs(ctx, hash, index, hash, index, numParticles, ctx->params.numCells, nullptr);
+  return sort_pairs(ctx, hash, index, hash, index, numParticles, ctx->params.numCells,
+                    stage_timer(ctx));
 }
 
 fc_status fc_reorder_data_and_find_cell_start(fc_context* ctx, uint32_t* cellStart,
@@ -378,14 +391,19 @@ fc_status fc_reorder_data_and_find_cell_start(fc_context* ctx, uint32_t* cellSta
   if (!ctx) return set_error(FC_ERROR_INVALID_VALUE, "ctx is NULL");
   if (numCells > 0 && (!cellStart || !cellEnd))
     return set_error(FC_ERROR_INVALID_VALUE, "NULL cell table");
+  StageTimer* t = stage_timer(ctx);
+  if (t) t->begin(ctx->stream, "drop_in_clear_tables");
   hipError_t e = launch_fill_u32(ctx->stream, cellStart, CELL_EMPTY, numCells);
   if (e == hipSuccess) e = launch_fill_u32(ctx->stream, cellEnd, CELL_EMPTY, numCells);
+  if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "clear cell tables");
   if (numParticles == 0) return FC_SUCCESS;
   if (!sortedPos || !sortedVel || !hash || !index || !oldPos || !oldVel)
     return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
+  if (t) t->begin(ctx->stream, "drop_in_reorder");
   e = launch_reorder(ctx->stream, cellStart, cellEnd, (float4*)sortedPos, (float4*)sortedVel, hash,
                      index, (const float4*)oldPos, (const float4*)oldVel, numParticles, numCells);
+  if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "reorderDataAndFindCellStart");
   return FC_SUCCESS;
 }
@@ -399,9 +417,12 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
     return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
   if (numCells != ctx->params.numCells)
     return set_error(FC_ERROR_INVALID_VALUE, "numCells differs from the context parameters");
+  StageTimer* t = stage_timer(ctx);
+  if (t) t->begin(ctx->stream, "drop_in_collide");
   hipError_t e = launch_collide(ctx->stream, ctx->params, false, ctx->devFlags, (float4*)newVel,
                                 (const float4*)sortedPos, (const float4*)sortedVel, index,
                                 cellStart, cellEnd, numParticles);
+  if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
 }

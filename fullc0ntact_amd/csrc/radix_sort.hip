@@ -319,26 +319,55 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   __syncthreads();
 
   const uint32_t validCount = (n - tileBase) < (uint32_t)RS_TILE ? (n - tileBase) : (uint32_t)RS_TILE;
+  // Output: every item's key, value and destination first, then (the last
+  // pass of a gathering sort) all of the items' state loads together, then
+  // the stores.  The state arrays may alias nothing here, but the compiler
+  // cannot know it: loads placed after a store wait for the store's round
+  // trip, which serialised the gather.
+  uint32_t ok[RS_ITEMS], ks[RS_ITEMS], vs[RS_ITEMS], gs[RS_ITEMS];
 #pragma unroll
   for (int i = 0; i < RS_ITEMS; ++i) {
     const uint32_t s = i * RS_THREADS + threadIdx.x;
+    ok[i] = 0u;
+    ks[i] = vs[i] = gs[i] = 0u;
     if (s < validCount) {
-      const uint32_t k = s_keys[s];
-      const uint32_t dg = (k >> shift) & (RS_RADIX - 1);
-      const uint32_t g = s_digitBase[dg] + s;
-      if (g < n) {  // always true unless a look-back stalled
-        const uint32_t v = s_vals[s];
-        keysOut[g] = k;
-        valsOut[g] = v;
-        if (ga.pos && v < n) {  // v < n: values index the particle arrays
-          const float4 pv = ga.pos[v];
-          ga.sortedPos[g] = pv;
-          const float4 vv = ga.vel[v];
-          ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
-          if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
-          if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv.x, pv.y, pv.z);
-        }
+      ks[i] = s_keys[s];
+      const uint32_t dg = (ks[i] >> shift) & (RS_RADIX - 1);
+      gs[i] = s_digitBase[dg] + s;
+      vs[i] = s_vals[s];
+      ok[i] = gs[i] < n ? 1u : 0u;  // always true unless a look-back stalled
+    }
+  }
+  if (ga.pos) {
+    float4 pv[RS_ITEMS], vv[RS_ITEMS];
+    uint32_t id[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const bool live = ok[i] && vs[i] < n;  // v < n: values index the particle arrays
+      const uint32_t v = live ? vs[i] : 0u;
+      pv[i] = ga.pos[v];
+      vv[i] = ga.vel[v];
+      id[i] = ga.idOfSlot ? ga.idOfSlot[v] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      if (!ok[i]) continue;
+      const uint32_t g = gs[i];
+      keysOut[g] = ks[i];
+      valsOut[g] = vs[i];
+      if (vs[i] < n) {
+        ga.sortedPos[g] = pv[i];
+        ga.sortedVel[g] = make_float4(vv[i].x, vv[i].y, vv[i].z, __uint_as_float(ks[i]));
+        if (ga.idOfSlot) ga.indexOut[g] = id[i];
+        if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv[i].x, pv[i].y, pv[i].z);
       }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      if (!ok[i]) continue;
+      keysOut[gs[i]] = ks[i];
+      valsOut[gs[i]] = vs[i];
     }
   }
   if (countNext) histogram_flush(s_nextHist, gHist, pass + 1);
@@ -453,22 +482,40 @@ __global__ __launch_bounds__(RS_THREADS) void k_sort_small(
   }
   // Sorted keys and values are in s_keys/s_vals[0, n); the clears above
   // completed before the first barrier, so the bounds below overwrite them.
+  // All items' state loads are issued before any store (see k_rs_scatter).
+  uint32_t ks[RS_ITEMS], vs[RS_ITEMS], kps[RS_ITEMS], id[RS_ITEMS];
+  float4 pv[RS_ITEMS], vv[RS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; ++i) {
+    const uint32_t g = i * RS_THREADS + threadIdx.x;
+    ks[i] = g < n ? s_keys[g] : 0u;
+    vs[i] = g < n ? s_vals[g] : 0u;
+    kps[i] = (g > 0 && g < n) ? s_keys[g - 1] : 0u;
+  }
+  if (ga.pos) {
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+      const uint32_t g = i * RS_THREADS + threadIdx.x;
+      const uint32_t v = (g < n && vs[i] < n) ? vs[i] : 0u;
+      pv[i] = ga.pos[v];
+      vv[i] = ga.vel[v];
+      id[i] = ga.idOfSlot ? ga.idOfSlot[v] : 0u;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < RS_ITEMS; ++i) {
     const uint32_t g = i * RS_THREADS + threadIdx.x;
     if (g < n) {
-      const uint32_t k = s_keys[g], v = s_vals[g];
+      const uint32_t k = ks[i], v = vs[i];
       keysOut[g] = k;
       valsOut[g] = v;
       if (ga.pos && v < n) {
-        const float4 pv = ga.pos[v];
-        ga.sortedPos[g] = pv;
-        const float4 vv = ga.vel[v];
-        ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
-        if (ga.idOfSlot) ga.indexOut[g] = ga.idOfSlot[v];
-        if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv.x, pv.y, pv.z);
+        ga.sortedPos[g] = pv[i];
+        ga.sortedVel[g] = make_float4(vv[i].x, vv[i].y, vv[i].z, __uint_as_float(k));
+        if (ga.idOfSlot) ga.indexOut[g] = id[i];
+        if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv[i].x, pv[i].y, pv[i].z);
       }
-      const uint32_t kp = g > 0 ? s_keys[g - 1] : 0u;
+      const uint32_t kp = kps[i];
       if (g == 0 || k != kp) {
         if (k < numCells) cellStart[k] = g;
         if (g > 0 && kp < numCells) cellEnd[kp] = g;

@@ -5,6 +5,7 @@
 #include "fc_internal.hpp"
 #include "distmap.hpp"
 #include "sdf_mlp.hpp"
+#include "radix_common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -480,10 +481,10 @@ __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast
 // queries are the columns, the layer outputs the rows, so every lane group
 // g = lane/16 holds rows 4g..4g+3 of each 16-row block for its query column
 // lane%16.  Returns the network output y for the lane's column.
-__device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
-                                               const float* __restrict__ P,
-                                               const float* __restrict__ F, float x, float y,
-                                               float z, int g) {
+template <class Frags>
+__device__ __forceinline__ float sdf_tile_mfma_f(const Frags& frag, const float* __restrict__ P,
+                                                 const float* __restrict__ F, float x, float y,
+                                                 float z, int g) {
   float sv[4], cv[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -494,7 +495,7 @@ __device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
   floatx4 d[4];
 #pragma unroll
   for (int ob = 0; ob < 4; ++ob)
-    d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag[ob], bop, floatx4{0, 0, 0, 0}, 0, 0, 0);
+    d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag(ob), bop, floatx4{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
   for (int L = 0; L < 2; ++L) {
     const int offB = L == 0 ? kSdfOffB1 : kSdfOffB2;
@@ -503,9 +504,9 @@ __device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
     bf16x8 in1 = act_operand(d[2], d[3], ld4(P + offB + 32 + 4 * g), ld4(P + offB + 48 + 4 * g));
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
-      floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag[f0 + 2 * ob], in0,
+      floatx4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag(f0 + 2 * ob), in0,
                                                             floatx4{0, 0, 0, 0}, 0, 0, 0);
-      d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag[f0 + 2 * ob + 1], in1, acc, 0, 0, 0);
+      d[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag(f0 + 2 * ob + 1), in1, acc, 0, 0, 0);
     }
   }
   float yv = 0.0f;
@@ -521,6 +522,26 @@ __device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
   yv += __shfl_xor(yv, 16);
   yv += __shfl_xor(yv, 32);
   return yv + P[kSdfOffB4];
+}
+
+// The tile with the network's fragments held in registers.
+__device__ __forceinline__ float sdf_tile_mfma(const bf16x8 (&frag)[kSdfFrags],
+                                               const float* __restrict__ P,
+                                               const float* __restrict__ F, float x, float y,
+                                               float z, int g) {
+  return sdf_tile_mfma_f([&](int f) { return frag[f]; }, P, F, x, y, z, g);
+}
+
+// The tile with each fragment loaded where it is used (the network's 20 KB
+// of packed weights stay in the L1/L2 caches across the tiles of a wave),
+// which frees the registers that hold them.
+__device__ __forceinline__ float sdf_tile_mfma_loaded(const uint16_t* __restrict__ packed,
+                                                      uint32_t net, int lane,
+                                                      const float* __restrict__ P,
+                                                      const float* __restrict__ F, float x,
+                                                      float y, float z, int g) {
+  return sdf_tile_mfma_f([&](int f) { return load_frag(packed, net, f, lane); }, P, F, x, y, z,
+                         g);
 }
 
 // Wave-level inference over 16-query tiles (grid-stride).  Queries of one
@@ -697,50 +718,61 @@ __global__ __launch_bounds__(kBT) void k_bucket_count(
     blockCounts[(size_t)k * gridDim.x + blockIdx.x] = s_cnt[k];
 }
 
-// Exclusive scan, in one block, of the per-block bucket counts laid out
-// bucket by bucket (blockCounts[k * blocks + b]): afterwards entry (k, b) is
-// where block b's spheres of bucket k start, so the buckets are contiguous
-// and ordered by block inside.  inBox = the start of the last bucket (the
-// spheres outside every box).  No global atomics: a few hundred blocks
-// adding to the same few dozen bucket counters serialise on them.
+// Exclusive scan of the per-block bucket counts, laid out bucket by bucket
+// (blockCounts[k * blocks + b]): one block per bucket k scans its row in
+// chunks of kScanThreads coalesced entries, so that afterwards entry (k, b)
+// is where block b's spheres start inside bucket k, and totals[k] holds the
+// bucket's size.  No global atomics: a few hundred blocks adding to the same
+// few dozen bucket counters serialise on them.
 constexpr uint32_t kScanThreads = 1024;
 
 __global__ __launch_bounds__(kScanThreads) void k_bucket_scan(uint32_t* __restrict__ blockCounts,
-                                                              uint32_t entries, uint32_t lastBucket,
-                                                              uint32_t* __restrict__ inBox) {
-  __shared__ uint32_t s_part[kScanThreads];
-  const uint32_t per = (entries + kScanThreads - 1) / kScanThreads;
-  const uint32_t lo = threadIdx.x * per, hi = lo + per < entries ? lo + per : entries;
-  uint32_t sum = 0;
-  for (uint32_t e = lo; e < hi; ++e) sum += blockCounts[e];
-  s_part[threadIdx.x] = sum;
-  __syncthreads();
-  for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // inclusive Hillis-Steele
-    const uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+                                                              uint32_t blocks,
+                                                              uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_scratch[kScanThreads / 64];
+  __shared__ uint32_t s_chunk;
+  uint32_t* row = blockCounts + (size_t)blockIdx.x * blocks;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < blocks; c0 += kScanThreads) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < blocks ? row[i] : 0u;
+    const uint32_t ex = scan_exclusive<kScanThreads>(v, s_scratch);
+    if (i < blocks) row[i] = carry + ex;
+    if (threadIdx.x == kScanThreads - 1) s_chunk = ex + v;
     __syncthreads();
-    s_part[threadIdx.x] += v;
+    carry += s_chunk;
     __syncthreads();
   }
-  uint32_t acc = threadIdx.x == 0 ? 0u : s_part[threadIdx.x - 1];
-  for (uint32_t e = lo; e < hi; ++e) {
-    const uint32_t c = blockCounts[e];
-    blockCounts[e] = acc;
-    if (e == lastBucket) *inBox = acc;
-    acc += c;
-  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// Scatter of the spheres into bucket order: the bucket bases are the
+// exclusive scan of the bucket totals (done by every block, in chunks of
+// kBT buckets), plus the block's start inside each bucket; inBox (written by
+// block 0) is the start of the last bucket, the spheres outside every box.
 __global__ __launch_bounds__(kBT) void k_bucket_scatter(const uint16_t* __restrict__ netOf,
                                                         uint32_t n, uint32_t nets,
                                                         const uint32_t* __restrict__ blockStart,
-                                                        uint32_t* __restrict__ order) {
+                                                        const uint32_t* __restrict__ totals,
+                                                        uint32_t* __restrict__ order,
+                                                        uint32_t* __restrict__ inBox) {
   extern __shared__ uint32_t s_cnt[];
   uint32_t* s_base = s_cnt + nets + 1;
-  for (uint32_t k = threadIdx.x; k <= nets; k += kBT) {
-    s_cnt[k] = 0u;
-    s_base[k] = blockStart[(size_t)k * gridDim.x + blockIdx.x];
+  __shared__ uint32_t s_scratch[kBT / 64];
+  __shared__ uint32_t s_chunk;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 <= nets; c0 += kBT) {
+    const uint32_t k = c0 + threadIdx.x;
+    const uint32_t v = k <= nets ? totals[k] : 0u;
+    const uint32_t ex = scan_exclusive<kBT>(v, s_scratch);
+    if (k <= nets) s_base[k] = carry + ex + blockStart[(size_t)k * gridDim.x + blockIdx.x];
+    if (k <= nets) s_cnt[k] = 0u;
+    if (k == nets && blockIdx.x == 0) *inBox = carry + ex;
+    if (threadIdx.x == kBT - 1) s_chunk = ex + v;
+    __syncthreads();
+    carry += s_chunk;
+    __syncthreads();
   }
-  __syncthreads();
   const uint32_t base = blockIdx.x * kBucketChunk;
 #pragma unroll
   for (uint32_t u = 0; u < kBucketChunk / kBT; ++u) {
@@ -751,6 +783,13 @@ __global__ __launch_bounds__(kBT) void k_bucket_scatter(const uint16_t* __restri
     }
   }
 }
+
+// Whether the cull loads each weight fragment where it is used (fewer
+// registers, more waves per SIMD) instead of holding all 20 per wave.
+#ifndef FC_CULL_LOADED_FRAGS
+#define FC_CULL_LOADED_FRAGS 1
+#endif
+constexpr bool kCullLoadedFrags = FC_CULL_LOADED_FRAGS != 0;
 
 // Surrogate cull over the bucketed spheres: the BF16 network predicts each
 // in-box sphere's distance, and those within radius + margin[body] are
@@ -797,7 +836,7 @@ __global__ __launch_bounds__(kTB) void k_boundary_cull(
       if (mask == 0) break;
       const int leadLane = __ffsll((unsigned long long)mask) - 1;
       const int lead = __shfl(net, leadLane);
-      if (lead != cur) {
+      if (!kCullLoadedFrags && lead != cur) {
 #pragma unroll
         for (int f = 0; f < kSdfFrags; ++f) frag[f] = load_frag(packed, (uint32_t)lead, f, lane);
         cur = lead;
@@ -807,7 +846,9 @@ __global__ __launch_bounds__(kTB) void k_boundary_cull(
       const float* fr = frame + (size_t)lead * 4;
       float x, y, z;
       sdf_norm(fr, q.x, q.y, q.z, x, y, z);
-      const float yv = sdf_tile_mfma(frag, P, F, x, y, z, g);
+      const float yv = kCullLoadedFrags
+                           ? sdf_tile_mfma_loaded(packed, (uint32_t)lead, lane, P, F, x, y, z, g)
+                           : sdf_tile_mfma(frag, P, F, x, y, z, g);
       const bool mine = remaining && net == lead;
       if (mine) cand = yv * fr[3] <= radius + margins[body];
       remaining = remaining && !mine;
@@ -1026,16 +1067,22 @@ void free_sdf(fc_sdf* m) {
 }
 
 constexpr uint32_t kTilesPerBlock = 8;
-constexpr uint32_t kMaxCullWaves = 2048;
+// Waves of the inference kernel (the 20 weight fragments in registers,
+// 170 VGPRs: two waves per SIMD of the 256 CUs) and of the cull (fragments
+// loaded where used, under 100 VGPRs: four waves per SIMD).
+constexpr uint32_t kMaxInferWaves = 2048;
+#ifndef FC_CULL_WAVES
+#define FC_CULL_WAVES 4096
+#endif
+constexpr uint32_t kMaxCullWaves = FC_CULL_WAVES;
 
 // Blocks (of 4 waves) of the MFMA kernels for n queries: at most the waves
-// that are resident at once (the kernels hold the 20 weight fragments in
-// registers, 170 VGPRs: two waves per SIMD of the 256 CUs), each wave
-// taking at least 16 tiles, so no wave waits for a second round.
-uint32_t mfma_blocks(uint32_t n) {
+// that are resident at once, each wave taking at least 16 tiles, so no wave
+// waits for a second round.
+uint32_t mfma_blocks(uint32_t n, uint32_t maxWaves = kMaxInferWaves) {
   const uint32_t tiles = (n + 15) / 16;
   uint32_t waves = (tiles + 15) / 16;
-  if (waves > kMaxCullWaves) waves = kMaxCullWaves;
+  if (waves > maxWaves) waves = maxWaves;
   const uint32_t blocks = (waves + 3) / 4;
   return blocks == 0 ? 1u : blocks;
 }
@@ -1396,7 +1443,7 @@ fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrog
     uint32_t* inBox = set->bins;
     uint32_t* waveCount = set->bins + 1;
     const unsigned blocks = (n + kBucketChunk - 1) / kBucketChunk;
-    const size_t entries = (size_t)(nets + 1) * blocks;
+    const size_t entries = (size_t)(nets + 1) * blocks + (nets + 1);  // + bucket totals
     if (set->blockBinsCap < entries) {
       FC_HIP_TRY(hipStreamSynchronize(st));
       hipFree(set->blockBins);
@@ -1410,13 +1457,14 @@ fc_status fc_boundary_contacts(fc_context* ctx, fc_body_set* set, fc_sdf* surrog
     hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(kBT), (nets + 1) * sizeof(uint32_t), st,
                        set->bodies, set->cellBody, L, set->B, (const float4*)pos, n, nets,
                        set->netOf, set->blockBins, candidate, contact);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(kScanThreads), 0, st, set->blockBins,
-                       (uint32_t)entries, (uint32_t)((size_t)nets * blocks), inBox);
+    uint32_t* totals = set->blockBins + (size_t)(nets + 1) * blocks;
+    hipLaunchKernelGGL(k_bucket_scan, dim3(nets + 1), dim3(kScanThreads), 0, st, set->blockBins,
+                       (uint32_t)blocks, totals);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks), dim3(kBT), 2 * (nets + 1) * sizeof(uint32_t),
-                       st, set->netOf, n, nets, set->blockBins, set->order);
+                       st, set->netOf, n, nets, set->blockBins, totals, set->order, inBox);
     if (timer) timer->end(st);
     FC_HIP_TRY(hipGetLastError());
-    const uint32_t cblocks = mfma_blocks(n);
+    const uint32_t cblocks = mfma_blocks(n, kMaxCullWaves);
     if (timer) timer->begin(st, "boundary_cull");
     hipLaunchKernelGGL(k_boundary_cull, dim3(cblocks), dim3(kTB), 0, st, surrogate->packed,
                        surrogate->params, surrogate->freq, surrogate->frame, set->bodies,

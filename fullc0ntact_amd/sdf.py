@@ -110,5 +110,6 @@ def training_samples(vertices, triangles, n: int, center, half_extent: float,
     surf += rng.normal(0.0, sigma * half_extent, size=surf.shape)
     uni = np.asarray(center) + rng.uniform(-half_extent, half_extent, size=(n - ns, 3))
     pts = np.concatenate([surf, uni])
-    rng.shuffle(pts)
-    return pts
+    # shuffled (the trainer takes minibatches of consecutive samples) by a
+    # permutation index: an in-place row shuffle is ~20x slower at 2^20 rows
+    return pts[rng.permutation(len(pts))]

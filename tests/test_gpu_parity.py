@@ -575,11 +575,12 @@ def test_system_list_overflow_fallback(gpu_ctx, poly):
     # row-range fallback, beside list-path lanes in the same waves.  Updates
     # and a graph-launched run must equal the oracle bit for bit.  With poly
     # the radii come from pos.w (U(0.6 r, r)), through the fallback's
-    # polydisperse branch.
+    # polydisperse branch.  Over 16384 spheres, so that the row-range
+    # kernel (not the row-split one of small systems) runs.
     from fullc0ntact_amd import ParticleSystem
     grid, r = 64, 1.0 / 64.0
     rng = np.random.default_rng(44)
-    parts = [rng.uniform(-0.95, 0.95, size=(2000, 3))]
+    parts = [rng.uniform(-0.95, 0.95, size=(20000, 3))]
     for _ in range(60):
         centre = rng.uniform(-0.9, 0.9, 3)
         parts.append(centre + rng.normal(scale=0.6 * r, size=(int(rng.integers(30, 61)), 3)))

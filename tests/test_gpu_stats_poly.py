@@ -126,9 +126,10 @@ def _poly_state(n, rmax, seed=11):
     return pos, vel
 
 
+@pytest.mark.parametrize("n", [8000, 24000])  # row-split and row-range collides
 @pytest.mark.parametrize("mode", ["update", "run"])
-def test_polydisperse_matches_oracle(gpu_ctx, mode):
-    n, grid, rmax = 8000, 64, 1.0 / 64
+def test_polydisperse_matches_oracle(gpu_ctx, mode, n):
+    grid, rmax = 64, 1.0 / 64
     pos, vel = _poly_state(n, rmax)
     ps = _system(gpu_ctx, n, grid, rmax)
     ps.setArray(ps.POSITION, pos, 0, n)
