@@ -130,15 +130,18 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
         # scattered back to id order by a run's last step
         return 60 * n + (16 * n if run else 0) + 8 * c_touch
     if name == "collide_integrate":  # own pos/vel 32, coordinates 8, index 4; pos/vel 32,
-        return 88 * n + 8 * c_touch  # key 4, slot 4, id of slot 4 (newVel fused away)
-    if name == "cell_bounds":  # sorted hashes 4 B, cellStart/cellEnd per occupied cell
-        return 4 * n + 8 * c_occ
-    if name.startswith("sort_pass"):  # keys+values read and written once; the last pass
-        # also gathers pos/vel into sorted order (32 B read, 32 B written), writes the
-        # fixed-point coordinates (8 B) and, in a run, the ids through the slot map (4 B
-        # read, 4 B written)
-        last = int(name[len("sort_pass"):]) == SORT_PASSES - 1
-        return 16 * n + ((72 * n + (8 * n if run else 0)) if last else 0)
+        return 84 * n + 8 * c_touch  # key 4 at the slot, id of slot 4 (newVel fused away)
+    if name == "cell_bounds":  # sorted hashes 4 B, cellStart/cellEnd per occupied cell; in
+        # a run also the ids 4 B (cells whose ids are out of order, a data-dependent
+        # share, are rewritten on top)
+        return (8 if run else 4) * n + 8 * c_occ
+    if name.startswith("sort_pass"):  # keys+values read and written once (the first pass
+        # reads keys only: its values are the keys' positions); the last pass also gathers pos/vel into
+        # sorted order (32 B read, 32 B written), writes the fixed-point coordinates
+        # (8 B) and, in a run, the ids through the slot map (4 B read, 4 B written)
+        p = int(name[len("sort_pass"):])
+        last = p == SORT_PASSES - 1
+        return (12 if p == 0 else 16) * n + ((72 * n + (8 * n if run else 0)) if last else 0)
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
         return 72 * n
     return None

@@ -164,8 +164,12 @@ hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t*
                              const SortGather& gather);
 // The cell-table half of reorderDataAndFindCellStart: cellStart/cellEnd of
 // the sorted hashes (the tables already hold EMPTY for every other cell).
+// With fixIndex..fixQ (all four): also puts each cell's run of the sorted
+// arrays into id order (for a sort of keys in slot order).
 hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
-                              const uint32_t* hash, uint32_t n, uint32_t numCells);
+                              const uint32_t* hash, uint32_t n, uint32_t numCells,
+                              uint32_t* fixIndex = nullptr, float4* fixPos = nullptr,
+                              float4* fixVel = nullptr, uint2* fixQ = nullptr);
 
 // ---------------------------------------------------------------------------
 // Per-stage timing with HIP events.
@@ -249,6 +253,9 @@ uint32_t quant_threshold(float R, float scale);
 // collide + the next update's integrate/hash/histogram in one launch (see
 // dem_kernels.hip); usable when collide_can_fuse(P, n).
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n);
+// Whether a run's fused collide stores the next key at the sorted slot
+// (then the sort takes keys in slot order and the cell bounds repair ids).
+bool collide_slot_keys(const fc_sim_params& P, uint32_t n);
 hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
                                 float4* pos, float4* vel,
                                 const float4* sortedPos, const float4* sortedVel,
@@ -256,7 +263,7 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
                                 uint32_t* slotOf, uint32_t* idOfSlot,
-                                const GatheredState* gathered);
+                                const GatheredState* gathered, bool slotKeys = false);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).

@@ -155,12 +155,17 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
 // which computes exactly what `steps` separate updates compute.
 // Between the fused steps the particle state stays in the previous sort's
 // order: the fused collide writes pos/vel at each particle's sorted slot k
-// (coalesced), the hash and slotOf[id] = k at the particle's id and
-// idOfSlot[k] = id.  The sort carries the slots as its values, in id order,
-// so its input and tie order are those of the separate updates; the next
-// reorder gathers pos/vel at the sorted slots (nearly in order) and writes
-// the ids through idOfSlot.  The tail's collide scatters the positions
-// (with the velocities) back to id order.
+// (coalesced) and idOfSlot[k] = id.  For large systems (collide_slot_keys)
+// it stores the next hash at the slot too: the sort takes the keys in slot
+// order with the slots as values, so each cell's run comes out in the
+// previous step's order, and the cell-bounds kernel then puts every run
+// into id order, the tie order of the separate updates.  Otherwise it
+// stores the hash and slotOf[id] = k at the particle's id and the sort
+// carries the slots as its values, in id order, so its input and tie order
+// are those of the separate updates.  Either way the next reorder gathers
+// pos/vel at the sorted slots (nearly in order) and writes the ids through
+// idOfSlot.  The tail's collide scatters the positions (with the
+// velocities) back to id order.
 int run_passes(const fc_sim_params& P) {
   const int p = radix_passes(key_bits_for_cells(P.numCells), false);
   return p < 2 ? 2 : p;
@@ -216,7 +221,11 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     ga.idOfSlot = sys->idOfSlot;
     ga.indexOut = sys->index;
   }
-  const uint32_t* valsIn = fromSlots ? sys->slotOf : nullptr;
+  // slotKeys: the fused collide stores the next keys at the sorted slots;
+  // keysAtSlots: this sort's keys were stored that way.
+  const bool slotKeys = collide_slot_keys(P, n);
+  const bool keysAtSlots = fromSlots && slotKeys;
+  const uint32_t* valsIn = (fromSlots && !keysAtSlots) ? sys->slotOf : nullptr;
   uint32_t* valsOut = fromSlots ? sys->slotOf : sys->index;
   const uint32_t* clearHash = clearInSort ? sys->hash : nullptr;
   hipError_t e;
@@ -233,7 +242,11 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
                          sys->cellStart, sys->cellEnd, &ga);
     if (e != hipSuccess) return hip_error(e, "sort");
     if (timer) timer->begin(st, "cell_bounds");
-    e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
+    if (keysAtSlots)
+      e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells,
+                             sys->index, sys->sortedPos, sys->sortedVel, sys->sortedQ);
+    else
+      e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "cell bounds");
   }
@@ -247,7 +260,7 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
                              sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf,
-                             sys->idOfSlot, &gs);
+                             sys->idOfSlot, &gs, slotKeys);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;

@@ -381,6 +381,34 @@ def test_attraction_matches_oracle(gpu_ctx, n):
         ps.close()
 
 
+@pytest.mark.parametrize("grid", [32, 64])
+def test_run_with_cell_mixing_matches_oracle(gpu_ctx, grid):
+    # Random (overlapping) start, 2 spheres per cell on the 32^3 grid: spheres
+    # change cells every few steps, so each cell's run leaves a fused step's
+    # sort in the previous step's order and must be put back in id order.
+    # 20 steps = head, graph replays and tail.
+    n, r = 65536, 1.0 / 64.0
+    print(f"[mixing] n={n} grid={grid}", flush=True)
+    from fullc0ntact_amd import ParticleSystem
+    ps = ParticleSystem(n, (grid, grid, grid))
+    P = sim_params(n, grid, r)
+    ps.m_params = P
+    pos, vel = orc.reset(n, 0, 77, r, 0.0, 0.02)
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    OP = orc.params_from(P)
+    ps.run(0.5, 20)
+    for _ in range(20):
+        st.update(0.5, OP)
+    _compare_system(ps, st)
+    ps.run(0.5, 3)
+    for _ in range(3):
+        st.update(0.5, OP)
+    _compare_system(ps, st)
+    ps.close()
+
+
 def test_full_size_steps_match_oracle(gpu_ctx):
     # 1,048,576 spheres r=1/128 on a 256^3 grid: two whole updates, bit-exact
     n, grid, r = 1 << 20, 256, 1.0 / 128.0
