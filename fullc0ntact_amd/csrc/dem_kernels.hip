@@ -10,6 +10,7 @@
 #include "radix_common.hpp"
 #include "distmap.hpp"
 #include "collide.hpp"
+#include "row_resort.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -153,188 +154,21 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 }
 
 // The cell-table half of k_reorder (the gather done by the sort's last
-// pass).  With fix.index (a sort of keys in slot order, whose ties follow
-// the previous step's (cell, id) order instead of the id): each cell's run
-// is also put into id order, permuting the gathered state with it (index,
-// sortedPos, sortedVel, sortedQ), which yields the (cell, id) order of the
-// reference's sort.  A wave owns the runs that start in its 64 entries and
-// also reads the kHalo entries after them, so a run that crosses its end
-// is seen whole unless it is longer than that.  The ballots say which
-// owned runs are out of order (a sphere changed cells); their entries are
-// permuted at once: each counts the smaller ids of its run with shuffles
-// and is written at that rank, after every load of the wave has completed.
-// A run that goes past the halo belongs to its first lane, which reads on
-// through memory and insertion-sorts it in place if it is out of order.
-// Runs are disjoint, so no entry has two writers; the ids of entries owned
-// by an earlier wave are read, and not used.
-struct CellFix {
-  uint32_t* index = nullptr;
-  float4* sortedPos = nullptr;
-  float4* sortedVel = nullptr;
-  uint2* sortedQ = nullptr;
-};
-
-constexpr uint32_t kHalo = 8;
-
-// Lanes a, a+1, ..., 63 of a wave (none when a >= 64).
-__device__ __forceinline__ uint64_t lanes_from(uint32_t a) {
-  return a >= 64u ? 0ull : (~0ull << a);
-}
-
-__device__ __forceinline__ uint32_t lowest_lane(uint64_t m) {
-  return (uint32_t)__ffsll((unsigned long long)m) - 1u;
-}
-
+// pass).
 __global__ __launch_bounds__(kBlock) void k_cell_bounds(uint32_t* __restrict__ cellStart,
                                                         uint32_t* __restrict__ cellEnd,
                                                         const uint32_t* __restrict__ hash,
-                                                        uint32_t n, uint32_t numCells,
-                                                        CellFix fix) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u;
-  const bool valid = k < n;
-  const bool fixing = fix.index != nullptr;  // the same for every lane
-  // every load the run checks need, issued before the table stores: the
-  // entry's hash and id, and halo entry `lane` (lanes below kHalo) with the
-  // hash after the halo
-  const uint32_t kh = k + 64u;
-  const bool hv = fixing && lane < kHalo && kh < n;
-  const uint32_t h = valid ? hash[k] : 0u;
-  const uint32_t hFirst = (lane == 0u && valid && k > 0) ? hash[k - 1] : 0u;
-  const uint32_t id = (fixing && valid) ? fix.index[k] : 0u;
-  const uint32_t hh = (fixing && lane <= kHalo && kh < n) ? hash[kh] : 0u;
-  const uint32_t idh = hv ? fix.index[kh] : 0u;
+                                                        uint32_t n, uint32_t numCells) {
+  uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t h = (k < n) ? hash[k] : 0u;
   uint32_t hprev = __shfl_up(h, 1);
-  if (lane == 0u) hprev = hFirst;
-  const bool start = valid && (k == 0 || h != hprev);
-  if (start) {
+  if (k >= n) return;
+  if ((threadIdx.x & 63u) == 0u && k > 0) hprev = hash[k - 1];
+  if (k == 0 || h != hprev) {
     if (h < numCells) cellStart[h] = k;
     if (k > 0 && hprev < numCells) cellEnd[hprev] = k;
   }
-  if (valid && k == n - 1 && h < numCells) cellEnd[h] = k + 1;
-  if (!fixing) return;
-
-  // run boundaries among the wave's entries
-  uint32_t hnext = __shfl_down(h, 1);
-  const uint32_t hh0 = __shfl(hh, 0);
-  if (lane == 63u) hnext = hh0;
-  const bool last = valid && (k + 1 == n || hnext != h);
-  const uint32_t idprev = __shfl_up(id, 1);
-  const bool inv = valid && !start && lane > 0u && id < idprev;
-  const uint64_t startMask = __ballot(start);
-  const uint64_t lastMask = __ballot(last);
-  const uint64_t invMask = __ballot(inv);
-  // ... and among the halo's, for the run that holds the wave's last entry
-  const uint32_t hhNext = __shfl_down(hh, 1);
-  const uint32_t id63 = __shfl(id, 63);
-  uint32_t idhPrev = __shfl_up(idh, 1);
-  if (lane == 0u) idhPrev = id63;
-  const uint32_t h63 = __shfl(h, 63);
-  const bool hMember = hv && hh == h63;  // the halo entry continues that run
-  const uint64_t hMemberMask = __ballot(hMember);
-  const bool hLast = hMember && (kh + 1 == n || hhNext != hh);
-  const uint64_t hLastMask = __ballot(hLast);
-  const uint64_t hInvMask = __ballot(hMember && idh < idhPrev);
-
-  // this lane's run: first lane s (if it starts in the wave) and last entry
-  // t, counted on past the wave into the halo (kUnbounded: beyond the halo)
-  constexpr uint32_t kUnbounded = 0xFFFFFFFFu;
-  const uint64_t before = startMask & ~lanes_from(lane + 1u);
-  const uint64_t after = lastMask & lanes_from(lane);
-  const uint32_t s = before ? 63u - (uint32_t)__clzll((long long)before) : 0u;
-  uint32_t t = after ? lowest_lane(after) : kUnbounded;
-  const bool crossing = valid && after == 0ull;  // holds the wave's last entry
-  // the run holding the wave's last entry goes on into the halo when the
-  // first halo entry continues it
-  const bool haloRun = (hMemberMask & 1ull) != 0ull;
-  // (a crossing lane's run always goes on into the halo: lane 63 is valid
-  // and not the last of its run, so entry 64 exists and continues it)
-  if (crossing && haloRun && hLastMask != 0ull) t = 64u + lowest_lane(hLastMask);
-  const bool owned = valid && before != 0ull && t != kUnbounded;
-  bool outOfOrder = (invMask & lanes_from(s + 1u) & ~lanes_from(t + 1u)) != 0ull;
-  if (t >= 64u && t != kUnbounded)
-    outOfOrder = outOfOrder || (hInvMask & ~lanes_from(t - 63u)) != 0ull;
-  const bool permute = owned && outOfOrder;
-  const uint64_t permuteMask = __ballot(permute);
-  // the halo entries of the crossing run, when that run is permuted
-  const bool crossPermute = (permuteMask >> 63) != 0ull && haloRun;
-  const uint32_t tc = __shfl(t, 63);
-  const uint32_t sc = __shfl(s, 63);
-  const bool hPermute = crossPermute && hMember && tc != kUnbounded && 64u + lane <= tc;
-  const uint64_t hPermuteMask = __ballot(hPermute);
-
-  if (permuteMask != 0ull) {  // the same for every lane
-    uint32_t rank = 0, rankH = 0;
-    for (uint64_t m = permuteMask; m != 0ull; m &= m - 1ull) {
-      const uint32_t l = lowest_lane(m);
-      const uint32_t other = __shfl(id, (int)l);
-      rank += (permute && l >= s && l <= t && other < id) ? 1u : 0u;
-      rankH += (hPermute && l >= sc && other < idh) ? 1u : 0u;
-    }
-    for (uint64_t m = hPermuteMask; m != 0ull; m &= m - 1ull) {
-      const uint32_t j = lowest_lane(m);
-      const uint32_t other = __shfl(idh, (int)j);
-      rank += (permute && t != kUnbounded && t >= 64u && 64u + j <= t && other < id) ? 1u : 0u;
-      rankH += (hPermute && other < idh) ? 1u : 0u;
-    }
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p, ph = p, vh = p;
-    uint2 q = make_uint2(0u, 0u), qh = q;
-    if (permute) {
-      p = fix.sortedPos[k];
-      v = fix.sortedVel[k];
-      q = fix.sortedQ[k];
-    }
-    if (hPermute) {
-      ph = fix.sortedPos[kh];
-      vh = fix.sortedVel[kh];
-      qh = fix.sortedQ[kh];
-    }
-    // every load of the wave completes before its first store
-    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (permute) {
-      const uint32_t dst = k - lane + s + rank;
-      fix.index[dst] = id;
-      fix.sortedPos[dst] = p;
-      fix.sortedVel[dst] = v;
-      fix.sortedQ[dst] = q;
-    }
-    if (hPermute) {
-      const uint32_t dst = k - lane + sc + rankH;
-      fix.index[dst] = idh;
-      fix.sortedPos[dst] = ph;
-      fix.sortedVel[dst] = vh;
-      fix.sortedQ[dst] = qh;
-    }
-  }
-  if (!(start && t == kUnbounded)) return;
-  // the first lane of a run that goes on past the halo: walk and sort it
-  uint32_t e = k + 1;
-  bool ordered = true;
-  uint32_t prevId = id;
-  while (e < n && hash[e] == h) {
-    const uint32_t nid = fix.index[e];
-    ordered = ordered && nid > prevId;
-    prevId = nid;
-    ++e;
-  }
-  if (ordered) return;
-  for (uint32_t i = k + 1; i < e; ++i) {
-    const uint32_t iid = fix.index[i];
-    const float4 p = fix.sortedPos[i], v = fix.sortedVel[i];
-    const uint2 q = fix.sortedQ[i];
-    uint32_t j = i;
-    while (j > k && fix.index[j - 1] > iid) {
-      fix.index[j] = fix.index[j - 1];
-      fix.sortedPos[j] = fix.sortedPos[j - 1];
-      fix.sortedVel[j] = fix.sortedVel[j - 1];
-      fix.sortedQ[j] = fix.sortedQ[j - 1];
-      --j;
-    }
-    fix.index[j] = iid;
-    fix.sortedPos[j] = p;
-    fix.sortedVel[j] = v;
-    fix.sortedQ[j] = q;
-  }
+  if (k == n - 1 && h < numCells) cellEnd[h] = k + 1;
 }
 
 // Sum of the pair forces of cells [s, e) on particle k, in index order,
@@ -441,8 +275,10 @@ __global__ __launch_bounds__(kBlock) void k_collide_v2(
 // (particledem_um.cu:538-589): no cursor sphere, and the force also stored
 // as float3 at the original index.
 // slotKeys (fused): the next key is stored at the sorted slot k (coalesced)
-// instead of the key at the id and the slot map, for a sort of the keys in
-// slot order whose cells' id order k_cell_bounds repairs.
+// instead of the key at the id and the slot map, for the row resort
+// (row_resort.hip), with kMovedBit when the cell changes; the per-row
+// counts it needs are accumulated here (rows) instead of the radix digit
+// histogram.
 template <bool kFuse, bool kPoly, bool kRows>
 __global__ __launch_bounds__(kBlock) void k_collide_v3(
     fc_sim_params P, BoundaryArg B, float contactD2, float4* __restrict__ newVel,
@@ -452,21 +288,26 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
     float dt, uint32_t* __restrict__ keysOut, int passes, uint32_t* gHist, uint32_t* counters,
     uint32_t* devFlags, uint32_t* __restrict__ slotOf, uint32_t* __restrict__ idOfSlot,
     v3f invCell, const uint2* __restrict__ sortedQ, uint32_t qThreshold,
-    float* __restrict__ forceOut, int slotKeys) {
+    float* __restrict__ forceOut, int slotKeys, RowHists rows) {
   __shared__ uint32_t s_list[kListCap + 1][kBlock];
   __shared__ uint32_t s_hist[kFuse ? RS_RADIX : 1];
   const uint32_t k = xcd_grouped_block(blockIdx.x, gridDim.x, xcdGroup) * kBlock + threadIdx.x;
   const bool valid = k < n;
-  if (kFuse) {
+  // rows.rowHist: the next sort is a row resort, which takes per-row counts
+  // instead of the radix digit histogram (the same for every block)
+  const bool digitHist = kFuse && rows.rowHist == nullptr;
+  if (digitHist) {
     for (int i = threadIdx.x; i < RS_RADIX; i += kBlock) s_hist[i] = 0u;
     histogram_producer_prologue(gHist, passes, counters);
   }
-  uint32_t key = 0;
+  uint32_t key = 0, oldKey = 0;
   if (valid) {
     const float4 p4 = sortedPos[k];
     const float rA = kPoly ? p4.w : P.particleRadius;
     if (kPoly && !(rA <= P.particleRadius)) atomicOr(devFlags, DEVERR_RADIUS);
-    const v3f pos0 = xyz(p4), vel0 = xyz(sortedVel[k]);
+    const float4 v4 = sortedVel[k];
+    oldKey = __float_as_uint(v4.w);  // the gather keeps the sorted hash in w
+    const v3f pos0 = xyz(p4), vel0 = xyz(v4);
     v3f force =
         kRows ? collide_one_rows<kPoly, kBlock>(P, invCell, contactD2, k, pos0, vel0, rA,
                                                 sortedPos, sortedVel, sortedQ, qThreshold,
@@ -492,7 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
       newVel[k] = make_float4(v.x, v.y, v.z, 0.0f);
       key = hash_of_q(P, cell_quotients(P, invCell, p));
       if (slotKeys) {
-        keysOut[k] = key;  // at the slot: the sort takes slot order, ids repaired per cell
+        // at the slot, with whether the sphere changed cell, for the row resort
+        keysOut[k] = key != oldKey ? (key | kMovedBit) : key;
       } else {
         keysOut[orig] = key;
         slotOf[orig] = k;
@@ -500,11 +342,13 @@ __global__ __launch_bounds__(kBlock) void k_collide_v3(
       idOfSlot[k] = orig;
     }
   }
-  if (kFuse) {
+  if (digitHist) {
     __syncthreads();
     histogram_add_digit(s_hist, key & (RS_RADIX - 1), valid);
     __syncthreads();
     histogram_flush(s_hist, gHist, 0);
+  } else if (kFuse) {
+    row_histograms(rows, k, key, oldKey, valid);
   }
 }
 
@@ -663,18 +507,10 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
 }
 
 hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
-                              const uint32_t* hash, uint32_t n, uint32_t numCells,
-                              uint32_t* fixIndex, float4* fixPos, float4* fixVel, uint2* fixQ) {
+                              const uint32_t* hash, uint32_t n, uint32_t numCells) {
   if (n == 0) return hipSuccess;
-  CellFix fix;
-  if (fixIndex && fixPos && fixVel && fixQ) {
-    fix.index = fixIndex;
-    fix.sortedPos = fixPos;
-    fix.sortedVel = fixVel;
-    fix.sortedQ = fixQ;
-  }
   hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, cellStart,
-                     cellEnd, hash, n, numCells, fix);
+                     cellEnd, hash, n, numCells);
   return hipGetLastError();
 }
 
@@ -744,7 +580,8 @@ static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                       const uint32_t* cellEnd, uint32_t n, float4* pos, float dt,
                       uint32_t* keysOut, int passes, uint32_t* gHist, uint32_t* counters,
                       uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
-                      float* forceOut = nullptr, bool slotKeys = false) {
+                      float* forceOut = nullptr, bool slotKeys = false,
+                      const RowHists* rows = nullptr) {
   if (gs != nullptr && rows_walk_ok(P) && n <= kSplitMaxSpheres) {
     const uint32_t waves = (n + kSplitSpheresPerWave - 1) / kSplitSpheresPerWave;
     hipLaunchKernelGGL((k_collide_split<kFuse, kPoly>), dim3((waves + 3) / 4), b, 0, s, P, B, d2,
@@ -757,12 +594,12 @@ static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
                        inv_cell(P), gs->sortedQ,
                        quant_threshold(P.particleRadius + P.particleRadius, gs->frame.scale),
-                       forceOut, slotKeys ? 1 : 0);
+                       forceOut, slotKeys ? 1 : 0, rows ? *rows : RowHists{});
   else
     hipLaunchKernelGGL((k_collide_v3<kFuse, kPoly, false>), g, b, 0, s, P, B, d2, newVel,
                        sortedPos, sortedVel, index, cellStart, cellEnd, n, kCollideXcdGroup, pos,
                        dt, keysOut, passes, gHist, counters, devFlags, slotOf, idOfSlot,
-                       inv_cell(P), nullptr, 0u, forceOut, 0);
+                       inv_cell(P), nullptr, 0u, forceOut, 0, RowHists{});
 }
 
 hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint32_t* devFlags,
@@ -792,14 +629,12 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
   return hipGetLastError();
 }
 
-// Whether a run's fused collide stores the next key at the sorted slot (the
-// row-range kernel; the row-split kernel of small systems keeps the key at
-// the id and the slot map).
-#ifndef FC_SLOT_KEYS
-#define FC_SLOT_KEYS 1
-#endif
+// Whether a run's fused collide stores the next key at the sorted slot for
+// a row resort (the row-range kernel on a grid whose row tables fit; the
+// row-split kernel of small systems keeps the key at the id and the slot
+// map, for the radix sort).
 bool collide_slot_keys(const fc_sim_params& P, uint32_t n) {
-  return FC_SLOT_KEYS != 0 && n > kSplitMaxSpheres && rows_walk_ok(P);
+  return n > kSplitMaxSpheres && rows_walk_ok(P) && row_resort_grid_ok(P);
 }
 
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n) {
@@ -813,19 +648,22 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
                                 uint32_t* slotOf, uint32_t* idOfSlot,
-                                const GatheredState* gathered, bool slotKeys) {
+                                const GatheredState* gathered, bool slotKeys,
+                                const RowHists* rows) {
   if (!collide_can_fuse(P, n)) return hipErrorInvalidValue;
+  // slot keys go with the row counts, for the row resort
+  if (slotKeys != (rows != nullptr)) return hipErrorInvalidValue;
   if (slotKeys && !(gathered != nullptr && collide_slot_keys(P, n))) return hipErrorInvalidValue;
   const float d2 = contact_d2_threshold(P.particleRadius + P.particleRadius);
   const dim3 g(grid_for(n, kBlock)), b(kBlock);
   if (B.poly)
     launch_v3<true, true>(s, gathered, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                           cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                          devFlags, slotOf, idOfSlot, nullptr, slotKeys);
+                          devFlags, slotOf, idOfSlot, nullptr, slotKeys, rows);
   else
     launch_v3<true, false>(s, gathered, g, b, P, B, d2, vel, sortedPos, sortedVel, index, cellStart,
                            cellEnd, n, pos, dt, keysOut, passes, sort.histogram, sort.counters,
-                           devFlags, slotOf, idOfSlot, nullptr, slotKeys);
+                           devFlags, slotOf, idOfSlot, nullptr, slotKeys, rows);
   return hipGetLastError();
 }
 

@@ -203,6 +203,9 @@ fc_status check_device_flags(fc_context* ctx) {
     return set_error(FC_ERROR_SORT_STALLED, "radix sort look-back exceeded its spin bound");
   if (flags & DEVERR_HIST_STALLED)
     return set_error(FC_ERROR_SORT_STALLED, "radix sort histogram never completed");
+  if (flags & DEVERR_ROW_COUNT)
+    return set_error(FC_ERROR_INVALID_VALUE,
+                     "row resort: a grid row's spheres and its counts disagree");
   if (flags & DEVERR_RADIUS)
     return set_error(FC_ERROR_INVALID_VALUE,
                      "a sphere radius (pos.w) is not <= particleRadius in polydisperse mode");

@@ -40,6 +40,7 @@ enum : uint32_t {
   DEVERR_SORT_KEY_RANGE = 1u << 1, // a key had bits above the sorted width
   DEVERR_HIST_STALLED = 1u << 2,   // histogram totals never reached n
   DEVERR_RADIUS = 1u << 3,         // polydisperse radius above particleRadius
+  DEVERR_ROW_COUNT = 1u << 4,      // row resort: a row's members and counts disagree
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
@@ -164,12 +165,8 @@ hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t*
                              const SortGather& gather);
 // The cell-table half of reorderDataAndFindCellStart: cellStart/cellEnd of
 // the sorted hashes (the tables already hold EMPTY for every other cell).
-// With fixIndex..fixQ (all four): also puts each cell's run of the sorted
-// arrays into id order (for a sort of keys in slot order).
 hipError_t launch_cell_bounds(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
-                              const uint32_t* hash, uint32_t n, uint32_t numCells,
-                              uint32_t* fixIndex = nullptr, float4* fixPos = nullptr,
-                              float4* fixVel = nullptr, uint2* fixQ = nullptr);
+                              const uint32_t* hash, uint32_t n, uint32_t numCells);
 
 // ---------------------------------------------------------------------------
 // Per-stage timing with HIP events.
@@ -214,6 +211,7 @@ hipError_t launch_integrate_hash(hipStream_t s, const fc_sim_params& P, const Bo
                                  const uint32_t* prevSortedHash, uint32_t* cellStart,
                                  uint32_t* cellEnd, uint32_t n, SortScratch& sort, int passes,
                                  uint32_t* devFlags);
+struct RowHists;  // row_resort.hpp
 hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
                                     float4* pos, float4* vel, float dt, uint32_t n);
 DistMapView distmap_view(const fc_distmap* m);
@@ -254,7 +252,7 @@ uint32_t quant_threshold(float R, float scale);
 // dem_kernels.hip); usable when collide_can_fuse(P, n).
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n);
 // Whether a run's fused collide stores the next key at the sorted slot
-// (then the sort takes keys in slot order and the cell bounds repair ids).
+// (then the run's later sorts are row resorts).
 bool collide_slot_keys(const fc_sim_params& P, uint32_t n);
 hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const BoundaryArg& B,
                                 float4* pos, float4* vel,
@@ -263,7 +261,8 @@ hipError_t launch_collide_fused(hipStream_t s, const fc_sim_params& P, const Bou
                                 const uint32_t* cellEnd, uint32_t n, float dt, uint32_t* keysOut,
                                 SortScratch& sort, int passes, uint32_t* devFlags,
                                 uint32_t* slotOf, uint32_t* idOfSlot,
-                                const GatheredState* gathered, bool slotKeys = false);
+                                const GatheredState* gathered, bool slotKeys = false,
+                                const RowHists* rows = nullptr);
 
 fc_status validate_params(const fc_sim_params& P);
 // Initial configurations of reset() into host float4 arrays (fc_system.hip).
@@ -293,6 +292,61 @@ struct fc_context {
   fc::StageTimer timer;
   double* statsScratch = nullptr; // FP64 reduction partials + histogram bins
 };
+
+namespace fc {
+// Tables of the row resort of a multi-step run (row_resort.hip): one
+// device block holding, per grid row (the gridSize.x cells sharing y, z),
+// the start of the row in the current and in the next sorted order, the
+// next keys' counts, and the bucket list of the spheres that change row.
+struct RowTables {
+  uint32_t rows = 0, shift = 0, groups = 0, capacity = 0;
+  uint32_t* block = nullptr;
+  uint32_t* startCur = nullptr;   // [rows + 1] row starts of the current order
+  uint32_t* startNext = nullptr;  // [rows + 1] row starts of the next order
+  uint32_t* chgStart = nullptr;   // [rows x copies + 1] bucket starts of the row changers
+  uint32_t* rowList = nullptr;    // [rows] the rows that hold spheres in the next order
+  uint32_t* groupSums = nullptr;  // [3][groups] per 256-row group: spheres, changers, rows
+  uint32_t* listCount = nullptr;  // [1] entries of rowList
+  uint32_t* zeroed = nullptr;     // the counts below, zero between steps
+  size_t zeroedWords = 0;
+  uint32_t* rowHist = nullptr;    // [rows x copies] next keys per row
+  uint32_t* chgHist = nullptr;    // [rows x copies] of those, the spheres whose row changes
+  uint32_t* chgCursor = nullptr;  // [rows x copies]
+  uint32_t* chgList = nullptr;    // [capacity] current slots of the row changers, by new row
+};
+
+// Where the row sort writes the sorted order: hash and id per position,
+// the state gathered from the sorted slots, and the cell tables.
+struct RowOutputs {
+  uint32_t* hash = nullptr;
+  uint32_t* index = nullptr;
+  const float4* pos = nullptr;
+  const float4* vel = nullptr;
+  float4* sortedPos = nullptr;
+  float4* sortedVel = nullptr;
+  uint2* sortedQ = nullptr;
+  uint32_t* cellStart = nullptr;
+  uint32_t* cellEnd = nullptr;
+  QuantFrame frame;
+};
+
+constexpr uint64_t kMaxRows = 1ull << 22;
+bool row_resort_grid_ok(const fc_sim_params& P);
+RowHists row_hists(const RowTables& T);
+fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n);
+void free_row_tables(RowTables& T);
+hipError_t launch_row_clear(hipStream_t s, const RowTables& T);
+hipError_t launch_row_scan(hipStream_t s, const RowTables& T);
+// Row starts of an order sorted by another path (the radix sort of a run's
+// first step), read from its sorted hashes.
+hipError_t launch_row_starts(hipStream_t s, const RowTables& T, const uint32_t* sortedHash,
+                             uint32_t n);
+hipError_t launch_row_changers(hipStream_t s, const RowTables& T, const uint32_t* keys,
+                               const uint32_t* hash, uint32_t* cellStart, uint32_t* cellEnd,
+                               uint32_t n, uint32_t numCells, uint32_t* devFlags);
+hipError_t launch_row_sort(hipStream_t s, const RowTables& T, const uint32_t* keys,
+                           const uint32_t* idOfSlot, const RowOutputs& O, uint32_t* devFlags);
+}  // namespace fc
 
 struct fc_system {
   fc_context* ctx = nullptr;
@@ -324,6 +378,9 @@ struct fc_system {
   const fc_distmap* boundaryMap = nullptr;
   // Radii from pos.w (set by fc_system_set_polydisperse).
   int polydisperse = 0;
+  // Row resort tables of multi-step runs (allocated by the first run that
+  // uses them).
+  fc::RowTables rowTables;
   fc::BoundaryArg boundary_arg() const;
   fc::QuantFrame quant_frame() const;
 };

@@ -2,6 +2,7 @@
 // particleSystem.h:22-136 / particleSystem.cpp:30-509): buffer ownership,
 // initial configurations, update(dt) and graph-launched multi-step runs.
 #include "fc_internal.hpp"
+#include "row_resort.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -58,6 +59,7 @@ void free_system_buffers(fc_system* sys) {
   sys->sortedQ = nullptr;
   sys->hash = sys->index = sys->keys = sys->slotOf = sys->idOfSlot = nullptr;
   sys->cellStart = sys->cellEnd = nullptr;
+  free_row_tables(sys->rowTables);
 }
 
 void drop_graph(fc_system* sys) {
@@ -155,21 +157,26 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
 // which computes exactly what `steps` separate updates compute.
 // Between the fused steps the particle state stays in the previous sort's
 // order: the fused collide writes pos/vel at each particle's sorted slot k
-// (coalesced) and idOfSlot[k] = id.  For large systems (collide_slot_keys)
-// it stores the next hash at the slot too: the sort takes the keys in slot
-// order with the slots as values, so each cell's run comes out in the
-// previous step's order, and the cell-bounds kernel then puts every run
-// into id order, the tie order of the separate updates.  Otherwise it
-// stores the hash and slotOf[id] = k at the particle's id and the sort
-// carries the slots as its values, in id order, so its input and tie order
-// are those of the separate updates.  Either way the next reorder gathers
-// pos/vel at the sorted slots (nearly in order) and writes the ids through
-// idOfSlot.  The tail's collide scatters the positions (with the
-// velocities) back to id order.
+// (coalesced) and idOfSlot[k] = id.  For large systems (run_row_mode) it
+// stores the next hash at the slot too, with the per-row counts, and the
+// next sort is a row resort (row_resort.hip), whose result is the (cell,
+// id) order of the separate updates; the run's first sort (keys by id,
+// from the head) is a radix sort whose cell-table pass also records the
+// row starts.  Otherwise it stores the hash and slotOf[id] = k at the
+// particle's id and the radix sort carries the slots as its values, in id
+// order, so its input and tie order are those of the separate updates.
+// Either way the next order's state is gathered from the sorted slots
+// (nearly in order) with the ids through idOfSlot.  The tail's collide
+// scatters the positions (with the velocities) back to id order.
 int run_passes(const fc_sim_params& P) {
   const int p = radix_passes(key_bits_for_cells(P.numCells), false);
   return p < 2 ? 2 : p;
 }
+
+// Whether a run's steps after its first sort resort the spheres row by row
+// (row_resort.hip) instead of radix-sorting them: then the fused collide
+// stores the next keys at the sorted slots.
+bool run_row_mode(const fc_system* sys) { return collide_slot_keys(sys->params, sys->n); }
 
 fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
   fc_context* ctx = sys->ctx;
@@ -187,6 +194,12 @@ fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
     if (e == hipSuccess) e = launch_fill_u32(st, sys->cellEnd, CELL_EMPTY, P.numCells);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "clear cell tables");
+  }
+  if (run_row_mode(sys)) {
+    // the row counts start from zero (a run that stopped early may have
+    // left counts behind)
+    e = launch_row_clear(st, sys->rowTables);
+    if (e != hipSuccess) return hip_error(e, "clear row counts");
   }
   if (timer) timer->begin(st, "integrate_hash");
   e = launch_integrate_hash(st, P, sys->boundary_arg(), sys->pos, sys->vel, dt, sys->keys, prev,
@@ -221,15 +234,42 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     ga.idOfSlot = sys->idOfSlot;
     ga.indexOut = sys->index;
   }
-  // slotKeys: the fused collide stores the next keys at the sorted slots;
-  // keysAtSlots: this sort's keys were stored that way.
-  const bool slotKeys = collide_slot_keys(P, n);
-  const bool keysAtSlots = fromSlots && slotKeys;
+  // rowMode: the fused collide stores the next keys at the sorted slots
+  // with per-row counts; keysAtSlots: this sort's keys were stored that way,
+  // so it is a row resort.
+  const bool rowMode = run_row_mode(sys);
+  const bool keysAtSlots = fromSlots && rowMode;
+  const RowHists rh = rowMode ? row_hists(sys->rowTables) : RowHists{};
   const uint32_t* valsIn = (fromSlots && !keysAtSlots) ? sys->slotOf : nullptr;
   uint32_t* valsOut = fromSlots ? sys->slotOf : sys->index;
   const uint32_t* clearHash = clearInSort ? sys->hash : nullptr;
   hipError_t e;
-  if (n <= (uint32_t)RS_TILE) {
+  if (rowMode && keysAtSlots) {
+    if (timer) timer->begin(st, "row_scan");
+    e = launch_row_scan(st, sys->rowTables);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "row scan");
+    if (timer) timer->begin(st, "row_changers");
+    e = launch_row_changers(st, sys->rowTables, sys->keys, sys->hash, sys->cellStart,
+                            sys->cellEnd, n, P.numCells, ctx->devFlags);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "row changers");
+    RowOutputs O;
+    O.hash = sys->hash;
+    O.index = sys->index;
+    O.pos = sys->pos;
+    O.vel = sys->vel;
+    O.sortedPos = sys->sortedPos;
+    O.sortedVel = sys->sortedVel;
+    O.sortedQ = sys->sortedQ;
+    O.cellStart = sys->cellStart;
+    O.cellEnd = sys->cellEnd;
+    O.frame = ga.frame;
+    if (timer) timer->begin(st, "row_sort");
+    e = launch_row_sort(st, sys->rowTables, sys->keys, sys->idOfSlot, O, ctx->devFlags);
+    if (timer) timer->end(st);
+    if (e != hipSuccess) return hip_error(e, "row sort");
+  } else if (n <= (uint32_t)RS_TILE) {
     // One block sorts, gathers and writes the cell tables.
     if (timer) timer->begin(st, "sort_small");
     e = launch_sort_small(st, ctx->sort, sys->keys, valsIn, sys->hash, valsOut, n, passes,
@@ -242,11 +282,9 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
                          sys->cellStart, sys->cellEnd, &ga);
     if (e != hipSuccess) return hip_error(e, "sort");
     if (timer) timer->begin(st, "cell_bounds");
-    if (keysAtSlots)
-      e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells,
-                             sys->index, sys->sortedPos, sys->sortedVel, sys->sortedQ);
-    else
-      e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
+    e = launch_cell_bounds(st, sys->cellStart, sys->cellEnd, sys->hash, n, P.numCells);
+    // the next step's row resort starts from this order's rows
+    if (e == hipSuccess && rowMode) e = launch_row_starts(st, sys->rowTables, sys->hash, n);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "cell bounds");
   }
@@ -260,7 +298,7 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     e = launch_collide_fused(st, P, sys->boundary_arg(), sys->pos, sys->vel, sys->sortedPos,
                              sys->sortedVel, sys->index, sys->cellStart, sys->cellEnd, n, dt,
                              sys->keys, ctx->sort, passes, ctx->devFlags, sys->slotOf,
-                             sys->idOfSlot, &gs, slotKeys);
+                             sys->idOfSlot, &gs, rowMode, rowMode ? &rh : nullptr);
   if (timer) timer->end(st);
   if (e != hipSuccess) return hip_error(e, "collide");
   return FC_SUCCESS;
@@ -510,6 +548,7 @@ fc_status fc_system_run(fc_system* sys, float deltaTime, uint32_t steps) {
     return FC_SUCCESS;
   }
   StageTimer* timer = ctx->timer.enabled ? &ctx->timer : nullptr;
+  if (run_row_mode(sys)) FC_TRY(ensure_row_tables(sys->rowTables, sys->params, sys->n));
   FC_TRY(enqueue_run_head(sys, deltaTime, timer));
   FC_TRY(enqueue_run_body(sys, deltaTime, /*clearInSort=*/false, /*last=*/false,
                           /*fromSlots=*/false, timer));

@@ -1,0 +1,625 @@
+// Row resort: the sort of a multi-step run's steady-state steps.
+//
+// Within a run the fused collide stores each sphere's next hash at its
+// sorted slot, so the keys arrive in the previous step's (cell, id) order
+// and only the spheres that changed cell are out of place (8-9 % per step
+// on a settled bed).  The result must be the order of the reference's
+// stable sort by hash of the ids (particleSystem.cu:265-270 sorts
+// (hash, index) pairs whose index is the identity), that is (cell, id).
+// A grid row (the gridSize.x cells that share y and z) is a contiguous
+// range of that order, so each row is sorted on its own:
+//
+//   k_row_group_sums  per 256-row group, the spheres, row changers and
+//                     occupied rows the key producer counted;
+//   k_row_scan        row starts of the next order (the current ones are
+//                     kept), the changers' bucket starts, and the list of
+//                     the occupied rows;
+//   k_row_changers    buckets the spheres whose row changes by their new
+//                     row; clears the cells of rows that empty;
+//   k_row_sort        a fixed grid over the occupied rows.  A row's members
+//                     are the spheres of its current range whose next key
+//                     stays in it plus its incomers.  Those that keep their
+//                     cell are already in (cell, id) order; the others (the
+//                     movers, few) are ranked among themselves, and each
+//                     member's place is its rank in its own list plus the
+//                     number of the other list below it.  Rows beyond LDS,
+//                     or with many movers, are ranked by counting.  The
+//                     members are written at the row's start with the
+//                     gathered state and the new cell tables.
+//
+// This replaces the radix passes and the cell-bounds kernel of those
+// steps; the first sort of a run (keys in id order) stays a radix sort.
+#include "row_resort.hpp"
+
+namespace fc {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kSortThreads = 256;
+constexpr uint32_t kRowCap = 1536;     // members of a row held in LDS (18 KB: 8 blocks a CU)
+constexpr uint32_t kMoverCap = 512;    // movers ranked by counting
+constexpr uint32_t kPerThread = kRowCap / kSortThreads;
+constexpr uint32_t kMoversPerThread = kMoverCap / kSortThreads;
+constexpr uint32_t kSortBlocks = 4096; // grid of the row sort
+static_assert(kRowCap % kSortThreads == 0 && kMoverCap % kSortThreads == 0, "whole per-thread shares");
+constexpr uint64_t kNoMember = ~0ull;
+static_assert(kScanThreads == (1 << kRowGroupShift), "one row per scan thread");
+
+// Exclusive scan of one value per thread over a 256-thread block; returns
+// the thread's prefix and writes the block total to *total.  s_wave holds
+// 4 words; the caller separates two uses of one s_wave by a barrier.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave,
+                                                         uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63u) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+    const uint32_t t = s_wave[w];
+    if (w < wave) before += t;
+    all += t;
+  }
+  *total = all;
+  return before + x - v;
+}
+
+// Sum of one value per thread over the block (every thread gets it).
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_wave) {
+  uint32_t total;
+  block_exclusive_scan(v, s_wave, &total);
+  return total;
+}
+
+__device__ __forceinline__ uint32_t* group_sums(const RowTables& T, int which) {
+  return T.groupSums + (size_t)which * T.groups;
+}
+
+// The kRowCopies counters of row r (16-byte aligned).
+__device__ __forceinline__ void load_copies(const uint32_t* a, uint32_t r, uint32_t (&c)[kRowCopies]) {
+  const uint4* p = reinterpret_cast<const uint4*>(a + (size_t)r * kRowCopies);
+  const uint4 lo = p[0], hi = p[1];
+  c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w;
+  c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
+}
+
+__device__ __forceinline__ void zero_copies(uint32_t* a, uint32_t r) {
+  uint4* p = reinterpret_cast<uint4*>(a + (size_t)r * kRowCopies);
+  p[0] = make_uint4(0u, 0u, 0u, 0u);
+  p[1] = make_uint4(0u, 0u, 0u, 0u);
+}
+static_assert(kRowCopies == 8, "two 16-byte words of counters per row");
+
+// One block per 256-row group, one row per thread: the group's spheres,
+// changers and occupied rows.
+__global__ __launch_bounds__(kScanThreads) void k_row_group_sums(RowTables T) {
+  __shared__ uint32_t s_wave[3][kScanThreads / 64];
+  const uint32_t r = (blockIdx.x << kRowGroupShift) + threadIdx.x;
+  uint32_t h = 0, c = 0;
+  if (r < T.rows) {
+    uint32_t x[kRowCopies], y[kRowCopies];
+    load_copies(T.rowHist, r, x);
+    load_copies(T.chgHist, r, y);
+#pragma unroll
+    for (uint32_t q = 0; q < kRowCopies; ++q) {
+      h += x[q];
+      c += y[q];
+    }
+  }
+  const uint32_t o = block_sum(h != 0u ? 1u : 0u, s_wave[2]);
+  h = block_sum(h, s_wave[0]);
+  c = block_sum(c, s_wave[1]);
+  if (threadIdx.x == 0) {
+    group_sums(T, 0)[blockIdx.x] = h;
+    group_sums(T, 1)[blockIdx.x] = c;
+    group_sums(T, 2)[blockIdx.x] = o;
+  }
+}
+
+// One block per 256-row group, one row per thread.  The group's bases are
+// the sums of the earlier groups; each row takes base + its prefix within
+// the group, and each (row, copy) its changers' bucket start.  Each row's
+// current start (startNext, written by the previous step) moves to
+// startCur before it is overwritten; occupied rows are listed in row
+// order; the counters are zeroed for the next producer.
+__global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T) {
+  __shared__ uint32_t s_wave[3][kScanThreads / 64];
+  const uint32_t g = blockIdx.x;
+  uint32_t bh = 0, bc = 0, bo = 0;
+  for (uint32_t i = threadIdx.x; i < g; i += kScanThreads) {
+    bh += group_sums(T, 0)[i];
+    bc += group_sums(T, 1)[i];
+    bo += group_sums(T, 2)[i];
+  }
+  bh = block_sum(bh, s_wave[0]);
+  bc = block_sum(bc, s_wave[1]);
+  bo = block_sum(bo, s_wave[2]);
+  __syncthreads();
+  const uint32_t r = (g << kRowGroupShift) + threadIdx.x;
+  const bool in = r < T.rows;
+  uint32_t x[kRowCopies], y[kRowCopies];
+  uint32_t h = 0, c = 0;
+  if (in) {
+    load_copies(T.rowHist, r, x);
+    load_copies(T.chgHist, r, y);
+#pragma unroll
+    for (uint32_t q = 0; q < kRowCopies; ++q) {
+      h += x[q];
+      c += y[q];
+    }
+  }
+  uint32_t th, tc, to;
+  const uint32_t ph = block_exclusive_scan(h, s_wave[0], &th) + bh;
+  uint32_t pc = block_exclusive_scan(c, s_wave[1], &tc) + bc;
+  const uint32_t po = block_exclusive_scan(h != 0u ? 1u : 0u, s_wave[2], &to) + bo;
+  if (in) {
+    T.startCur[r] = T.startNext[r];
+    T.startNext[r] = ph;
+    if (h != 0u) T.rowList[po] = r;
+    uint32_t st[kRowCopies];
+#pragma unroll
+    for (uint32_t q = 0; q < kRowCopies; ++q) {
+      st[q] = pc;
+      pc += y[q];
+    }
+    uint4* cs = reinterpret_cast<uint4*>(T.chgStart + (size_t)r * kRowCopies);
+    cs[0] = make_uint4(st[0], st[1], st[2], st[3]);
+    cs[1] = make_uint4(st[4], st[5], st[6], st[7]);
+    zero_copies(T.rowHist, r);
+    zero_copies(T.chgHist, r);
+    zero_copies(T.chgCursor, r);
+  }
+  if (g == T.groups - 1u && threadIdx.x == kScanThreads - 1) {
+    // the last thread's prefixes plus its own counts are the totals
+    T.startCur[T.rows] = T.startNext[T.rows];
+    T.startNext[T.rows] = ph + h;  // past the last row: the sphere count
+    T.chgStart[(size_t)T.rows * kRowCopies] = pc;
+    *T.listCount = po + (h != 0u ? 1u : 0u);
+  }
+}
+
+// Row starts of a sorted hash array: the first position whose hash lies in
+// row r or beyond (a binary search per row).
+__global__ __launch_bounds__(256) void k_row_starts(RowTables T, const uint32_t* __restrict__ hash,
+                                                    uint32_t n) {
+  const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+  if (r > T.rows) return;
+  uint32_t lo = 0, hi = n;
+  if (r == T.rows) {
+    lo = n;
+  } else {
+    const uint32_t first = r << T.shift;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (hash[mid] < first)
+        lo = mid + 1u;
+      else
+        hi = mid;
+    }
+  }
+  T.startNext[r] = lo;
+}
+
+// One thread per current sorted entry k (keys[k] is its next key with
+// kMovedBit when the cell changes, hash[k] its current one).  A sphere
+// whose row changes is listed in its new row's bucket.  A current cell
+// none of whose spheres stays is cleared in the tables (the row sort, which
+// runs next, writes every cell of the next order, this one too if
+// spheres move in).
+__global__ __launch_bounds__(256) void k_row_changers(RowTables T, const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ hash,
+                                                      uint32_t* __restrict__ cellStart,
+                                                      uint32_t* __restrict__ cellEnd, uint32_t n,
+                                                      uint32_t numCells, uint32_t* devFlags) {
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t raw = keys[k], old = hash[k];
+  const uint32_t key = raw & ~kMovedBit;
+  if (k == 0 || hash[k - 1] != old) {
+    bool stays = (raw & kMovedBit) == 0u;
+    for (uint32_t k2 = k + 1; !stays && k2 < n && hash[k2] == old; ++k2)
+      stays = (keys[k2] & kMovedBit) == 0u;
+    if (!stays && old < numCells) {
+      cellStart[old] = CELL_EMPTY;
+      cellEnd[old] = CELL_EMPTY;
+    }
+  }
+  const uint32_t oldRow = old >> T.shift;
+  const uint32_t r = key >> T.shift;
+  if (r == oldRow) return;
+  if (r >= T.rows) {
+    atomicOr(devFlags, DEVERR_ROW_COUNT);
+    return;
+  }
+  const uint32_t b = r * kRowCopies + row_copy(k);
+  const uint32_t at = T.chgStart[b] + atomicAdd(&T.chgCursor[b], 1u);
+  if (at < T.chgStart[b + 1])
+    T.chgList[at] = k;
+  else
+    atomicOr(devFlags, DEVERR_ROW_COUNT);
+}
+
+// Writes a member at position q: the sorted hash and id, the gathered
+// state, and the cell-table entries where the cell changes.
+__device__ __forceinline__ void emit_member(const RowOutputs& O, uint32_t q, uint32_t key,
+                                            uint32_t id, uint32_t slot, bool firstOfCell,
+                                            bool lastOfCell) {
+  O.hash[q] = key;
+  O.index[q] = id;
+  const float4 p = O.pos[slot];
+  const float4 v = O.vel[slot];
+  O.sortedPos[q] = p;
+  O.sortedVel[q] = make_float4(v.x, v.y, v.z, __uint_as_float(key));
+  O.sortedQ[q] = quantize_pos(O.frame, p.x, p.y, p.z);
+  if (firstOfCell) O.cellStart[key] = q;
+  if (lastOfCell) O.cellEnd[key] = q + 1u;
+}
+
+// Writes the row's m members, in order in s_key/s_slot, from position o0:
+// each thread loads the state of up to kEmitBatch members before it stores
+// any, so the gathers' latencies overlap.
+constexpr uint32_t kEmitBatch = 4;
+__device__ __forceinline__ void emit_row(const RowOutputs& O, const uint64_t* s_key,
+                                         const uint32_t* s_slot, uint32_t o0, uint32_t m,
+                                         uint32_t rowBase) {
+  for (uint32_t base = 0; base < m; base += kSortThreads * kEmitBatch) {
+    float4 p[kEmitBatch], v[kEmitBatch];
+#pragma unroll
+    for (uint32_t t = 0; t < kEmitBatch; ++t) {
+      const uint32_t i = base + t * kSortThreads + threadIdx.x;
+      if (i < m) {
+        const uint32_t slot = s_slot[i];
+        p[t] = O.pos[slot];
+        v[t] = O.vel[slot];
+      }
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kEmitBatch; ++t) {
+      const uint32_t i = base + t * kSortThreads + threadIdx.x;
+      if (i >= m) continue;
+      const uint64_t ck = s_key[i];
+      const uint32_t x = (uint32_t)(ck >> 32);
+      const uint32_t key = rowBase | x;
+      const uint32_t q = o0 + i;
+      O.hash[q] = key;
+      O.index[q] = (uint32_t)ck;
+      O.sortedPos[q] = p[t];
+      O.sortedVel[q] = make_float4(v[t].x, v[t].y, v[t].z, __uint_as_float(key));
+      O.sortedQ[q] = quantize_pos(O.frame, p[t].x, p[t].y, p[t].z);
+      if (i == 0u || (uint32_t)(s_key[i - 1] >> 32) != x) O.cellStart[key] = q;
+      if (i + 1u == m || (uint32_t)(s_key[i + 1] >> 32) != x) O.cellEnd[key] = q + 1u;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t member_key(uint32_t key, uint32_t shift, uint32_t id) {
+  return ((uint64_t)(key & ((1u << shift) - 1u)) << 32) | id;
+}
+
+// Candidate u of row r: entry a0 + u of the current order for u < A (a
+// member if its next key stays in the row), else incomer u - A.  Returns
+// the member's (cell in row, id) key, or kNoMember.
+__device__ __forceinline__ uint64_t candidate(const RowTables& T, const uint32_t* keys,
+                                              const uint32_t* idOfSlot, uint32_t r, uint32_t a0,
+                                              uint32_t A, uint32_t c0, uint32_t u,
+                                              uint32_t* slotOut) {
+  const uint32_t k = u < A ? a0 + u : T.chgList[c0 + (u - A)];
+  const uint32_t key = keys[k] & ~kMovedBit;
+  *slotOut = k;
+  if ((key >> T.shift) != r) return kNoMember;
+  return member_key(key, T.shift, idOfSlot[k]);
+}
+
+// Number of entries of the ascending s_key[lo, lo + len) below x.
+__device__ __forceinline__ uint32_t count_below(const uint64_t* s_key, uint32_t lo, uint32_t len,
+                                                uint64_t x) {
+  uint32_t a = 0, b = len;
+  while (a < b) {
+    const uint32_t mid = (a + b) >> 1;
+    if (s_key[lo + mid] < x)
+      a = mid + 1u;
+    else
+      b = mid;
+  }
+  return a;
+}
+
+// The members of row r (more than LDS holds, or too many movers to rank):
+// each member's place is the number of members below it, counted over the
+// candidates staged through LDS in kRowCap chunks.  O(m x candidates).
+__device__ void rank_by_counting(const RowTables& T, const uint32_t* keys,
+                                 const uint32_t* idOfSlot, const RowOutputs& O, uint32_t r,
+                                 uint32_t o0, uint32_t m, uint32_t a0, uint32_t A, uint32_t c0,
+                                 uint32_t C, uint64_t* s_key, uint32_t* s_wave,
+                                 uint32_t* devFlags) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t rowBase = r << T.shift;
+  const uint32_t U = A + C;
+  uint32_t members = 0;
+  for (uint32_t tb = 0; tb < U; tb += kSortThreads) {
+    const uint32_t u = tb + tid;
+    uint32_t slot = 0;
+    const uint64_t mine = u < U ? candidate(T, keys, idOfSlot, r, a0, A, c0, u, &slot) : kNoMember;
+    uint32_t below = 0, sameBelow = 0, sameAbove = 0;
+    for (uint32_t w = 0; w < U; w += kRowCap) {
+      __syncthreads();
+      for (uint32_t i = tid; i < kRowCap; i += kSortThreads) {
+        uint32_t sl;
+        s_key[i] =
+            w + i < U ? candidate(T, keys, idOfSlot, r, a0, A, c0, w + i, &sl) : kNoMember;
+      }
+      __syncthreads();
+      if (mine != kNoMember) {
+        const uint32_t xm = (uint32_t)(mine >> 32);
+        for (uint32_t i = 0; i < kRowCap; ++i) {
+          const uint64_t o = s_key[i];
+          if (o == kNoMember) continue;
+          below += o < mine ? 1u : 0u;
+          const bool same = (uint32_t)(o >> 32) == xm;
+          sameBelow += (same && o < mine) ? 1u : 0u;
+          sameAbove += (same && o > mine) ? 1u : 0u;
+        }
+      }
+    }
+    if (mine != kNoMember) {
+      ++members;
+      if (below < m)
+        emit_member(O, o0 + below, rowBase | (uint32_t)(mine >> 32), (uint32_t)mine, slot,
+                    sameBelow == 0u, sameAbove == 0u);
+      else
+        atomicOr(devFlags, DEVERR_ROW_COUNT);
+    }
+  }
+  if (block_sum(members, s_wave) != m && tid == 0) atomicOr(devFlags, DEVERR_ROW_COUNT);
+}
+
+// The next keys carry kMovedBit; the sorted hashes are written to O.hash
+// while other blocks still read their rows, so the current hashes are not
+// read here.
+__global__ __launch_bounds__(kSortThreads) void k_row_sort(RowTables T,
+                                                           const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ idOfSlot,
+                                                           RowOutputs O, uint32_t* devFlags) {
+  // pool: the stayers in order from the front, the movers from the back;
+  // after the merge the row's members in order from the front
+  __shared__ uint64_t s_key[kRowCap];
+  __shared__ uint32_t s_slot[kRowCap];
+  __shared__ uint32_t s_wave[kScanThreads / 64];
+  __shared__ uint32_t s_movers;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t listed = *T.listCount;
+  for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
+    const uint32_t r = T.rowList[j];
+    const uint32_t o0 = T.startNext[r], m = T.startNext[r + 1] - o0;
+    const uint32_t a0 = T.startCur[r], A = T.startCur[r + 1] - a0;
+    const uint32_t c0 = T.chgStart[r * kRowCopies], C = T.chgStart[(r + 1) * kRowCopies] - c0;
+    const uint32_t rowBase = r << T.shift;
+    __syncthreads();  // the previous row's LDS reads are done
+    if (m > kRowCap) {
+      rank_by_counting(T, keys, idOfSlot, O, r, o0, m, a0, A, c0, C, s_key, s_wave, devFlags);
+      continue;
+    }
+    // gather: stayers (same cell) in order at the front, movers at the back;
+    // each thread takes a contiguous segment of the current range, counts
+    // its stayers, and after one scan places its members
+    if (tid == 0) s_movers = 0u;
+    const uint32_t E = (A + kSortThreads - 1u) / kSortThreads;
+    const uint32_t u0 = min(A, tid * E), u1 = min(A, u0 + E);
+    uint32_t mine = 0;
+#pragma unroll 4
+    for (uint32_t u = u0; u < u1; ++u) {
+      const uint32_t raw = keys[a0 + u];
+      mine += (((raw & ~kMovedBit) >> T.shift) == r && (raw & kMovedBit) == 0u) ? 1u : 0u;
+    }
+    uint32_t stayers;  // the same in every thread
+    uint32_t at = block_exclusive_scan(mine, s_wave, &stayers);
+#pragma unroll 4
+    for (uint32_t u = u0; u < u1; ++u) {
+      const uint32_t k = a0 + u;
+      const uint32_t raw = keys[k], id = idOfSlot[k];
+      const uint32_t key = raw & ~kMovedBit;
+      if ((key >> T.shift) != r) continue;
+      const uint32_t pos =
+          (raw & kMovedBit) == 0u ? at++ : kRowCap - 1u - atomicAdd(&s_movers, 1u);
+      if (pos < kRowCap) {  // (beyond it the counts disagree: checked below)
+        s_key[pos] = member_key(key, T.shift, id);
+        s_slot[pos] = k;
+      }
+    }
+    for (uint32_t i = tid; i < C; i += kSortThreads) {
+      const uint32_t k = T.chgList[c0 + i];
+      const uint32_t pos = kRowCap - 1u - atomicAdd(&s_movers, 1u);
+      if (pos < kRowCap) {
+        s_key[pos] = member_key(keys[k] & ~kMovedBit, T.shift, idOfSlot[k]);
+        s_slot[pos] = k;
+      }
+    }
+    __syncthreads();
+    const uint32_t movers = s_movers;
+    if (stayers + movers != m) {  // the counts and the members disagree: write nothing
+      if (tid == 0) atomicOr(devFlags, DEVERR_ROW_COUNT);
+      continue;
+    }
+    const uint32_t mBase = kRowCap - movers;  // movers at [mBase, kRowCap)
+    if (movers <= kMoverCap) {
+      // rank the movers among themselves and put them in order
+      uint64_t mk[kMoversPerThread];
+      uint32_t ms[kMoversPerThread], mr[kMoversPerThread];
+#pragma unroll
+      for (uint32_t t = 0; t < kMoversPerThread; ++t) {
+        const uint32_t i = tid + t * kSortThreads;
+        mk[t] = kNoMember;
+        ms[t] = 0u;
+        mr[t] = 0u;
+        if (i < movers) {
+          mk[t] = s_key[mBase + i];
+          ms[t] = s_slot[mBase + i];
+          uint32_t below = 0;
+          for (uint32_t q = 0; q < movers; ++q) below += s_key[mBase + q] < mk[t] ? 1u : 0u;
+          mr[t] = below;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t t = 0; t < kMoversPerThread; ++t)
+        if (tid + t * kSortThreads < movers) {
+          s_key[mBase + mr[t]] = mk[t];
+          s_slot[mBase + mr[t]] = ms[t];
+        }
+      __syncthreads();
+      // each member's place: its rank in its list + the other list's below it
+      uint64_t sk[kPerThread];
+      uint32_t ss[kPerThread], sr[kPerThread];
+#pragma unroll
+      for (uint32_t t = 0; t < kPerThread; ++t) {
+        const uint32_t i = tid + t * kSortThreads;
+        sk[t] = kNoMember;
+        ss[t] = 0u;
+        sr[t] = 0u;
+        if (i < stayers) {
+          sk[t] = s_key[i];
+          ss[t] = s_slot[i];
+          sr[t] = i + count_below(s_key, mBase, movers, sk[t]);
+        }
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < kMoversPerThread; ++t) {
+        const uint32_t i = tid + t * kSortThreads;
+        if (i < movers) {
+          mk[t] = s_key[mBase + i];
+          ms[t] = s_slot[mBase + i];
+          mr[t] = i + count_below(s_key, 0u, stayers, mk[t]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t t = 0; t < kPerThread; ++t)
+        if (tid + t * kSortThreads < stayers) {
+          s_key[sr[t]] = sk[t];
+          s_slot[sr[t]] = ss[t];
+        }
+#pragma unroll
+      for (uint32_t t = 0; t < kMoversPerThread; ++t)
+        if (tid + t * kSortThreads < movers) {
+          s_key[mr[t]] = mk[t];
+          s_slot[mr[t]] = ms[t];
+        }
+    } else {
+      // many movers (a transient): count instead
+      rank_by_counting(T, keys, idOfSlot, O, r, o0, m, a0, A, c0, C, s_key, s_wave, devFlags);
+      continue;
+    }
+    __syncthreads();
+    emit_row(O, s_key, s_slot, o0, m, rowBase);
+  }
+}
+
+}  // namespace
+
+bool row_resort_grid_ok(const fc_sim_params& P) {
+  const uint64_t rows = (uint64_t)P.gridSize.y * P.gridSize.z;
+  return rows >= 1 && rows <= kMaxRows && P.gridSize.x <= (1u << 20);
+}
+
+RowHists row_hists(const RowTables& T) {
+  RowHists h;
+  h.rowHist = T.rowHist;
+  h.chgHist = T.chgHist;
+  h.shift = T.shift;
+  return h;
+}
+
+fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n) {
+  uint32_t shift = 0;
+  while ((1u << shift) < P.gridSize.x) ++shift;
+  const uint32_t rows = P.gridSize.y * P.gridSize.z;
+  const uint32_t groups = (rows + (1u << kRowGroupShift) - 1u) >> kRowGroupShift;
+  if (T.block != nullptr && T.rows == rows && T.shift == shift && T.capacity >= n)
+    return FC_SUCCESS;
+  free_row_tables(T);
+  // startCur, startNext [rows + 1]; chgStart [rows x copies + 1]; rowList
+  // [rows]; groupSums [3 groups]; listCount [1]; rowHist, chgHist,
+  // chgCursor [rows x copies] (zeroed); chgList [n]
+  // every array starts on a 64-word (256-byte) boundary
+  const size_t cells = (size_t)rows * kRowCopies;
+  auto up = [](size_t x) { return (x + 63) & ~(size_t)63; };
+  const size_t zeroWords = 3 * cells;
+  const size_t words = 2 * up(rows + 1) + up(cells + 1) + up(rows) + up(3 * (size_t)groups) +
+                       up(1) + zeroWords + (n ? n : 1);
+  FC_HIP_TRY(hipMalloc(&T.block, words * sizeof(uint32_t)));
+  uint32_t* w = T.block;
+  T.startCur = w;
+  w += up(rows + 1);
+  T.startNext = w;
+  w += up(rows + 1);
+  T.chgStart = w;
+  w += up(cells + 1);
+  T.rowList = w;
+  w += up(rows);
+  T.groupSums = w;
+  w += up(3 * (size_t)groups);
+  T.listCount = w;
+  w += up(1);
+  T.zeroed = w;
+  T.zeroedWords = zeroWords;
+  T.rowHist = w;
+  w += cells;
+  T.chgHist = w;
+  w += cells;
+  T.chgCursor = w;
+  w += cells;
+  T.chgList = w;
+  T.rows = rows;
+  T.shift = shift;
+  T.groups = groups;
+  T.capacity = n;
+  FC_HIP_TRY(hipMemset(T.block, 0, words * sizeof(uint32_t)));
+  return FC_SUCCESS;
+}
+
+void free_row_tables(RowTables& T) {
+  if (T.block) hipFree(T.block);
+  T = RowTables{};
+}
+
+hipError_t launch_row_clear(hipStream_t s, const RowTables& T) {
+  return launch_fill_u32(s, T.zeroed, 0u, T.zeroedWords);
+}
+
+hipError_t launch_row_scan(hipStream_t s, const RowTables& T) {
+  hipLaunchKernelGGL(k_row_group_sums, dim3(T.groups), dim3(kScanThreads), 0, s, T);
+  hipLaunchKernelGGL(k_row_scan, dim3(T.groups), dim3(kScanThreads), 0, s, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_starts(hipStream_t s, const RowTables& T, const uint32_t* sortedHash,
+                             uint32_t n) {
+  hipLaunchKernelGGL(k_row_starts, dim3((T.rows + 1u + 255u) / 256u), dim3(256), 0, s, T,
+                     sortedHash, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_changers(hipStream_t s, const RowTables& T, const uint32_t* keys,
+                               const uint32_t* hash, uint32_t* cellStart, uint32_t* cellEnd,
+                               uint32_t n, uint32_t numCells, uint32_t* devFlags) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_changers, dim3((n + 255u) / 256u), dim3(256), 0, s, T, keys, hash,
+                     cellStart, cellEnd, n, numCells, devFlags);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_sort(hipStream_t s, const RowTables& T, const uint32_t* keys,
+                           const uint32_t* idOfSlot, const RowOutputs& O, uint32_t* devFlags) {
+  const uint32_t blocks = T.rows < kSortBlocks ? T.rows : kSortBlocks;
+  hipLaunchKernelGGL(k_row_sort, dim3(blocks), dim3(kSortThreads), 0, s, T, keys, idOfSlot, O,
+                     devFlags);
+  return hipGetLastError();
+}
+
+}  // namespace fc

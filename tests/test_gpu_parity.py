@@ -409,6 +409,38 @@ def test_run_with_cell_mixing_matches_oracle(gpu_ctx, grid):
     ps.close()
 
 
+@pytest.mark.parametrize("rows", [1, 8, 16])
+def test_run_with_crowded_rows_matches_oracle(gpu_ctx, rows):
+    # 20,000 spheres packed into `rows` grid rows of a 32^3 grid (x spread
+    # over the row, y and z inside one cell per row): 20000, 2500 and 1250
+    # spheres per row, above and below what a row sorts in LDS; the packed
+    # spheres push apart, so many change row every step.
+    n, grid, r = 20_000, 32, 1.0 / 64.0
+    print(f"[crowded] n={n} rows={rows}", flush=True)
+    from fullc0ntact_amd import ParticleSystem
+    ps = ParticleSystem(n, (grid, grid, grid))
+    P = sim_params(n, grid, r)
+    ps.m_params = P
+    rng = np.random.default_rng(31 + rows)
+    cell = 2.0 / grid
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 3] = 1.0
+    which = rng.integers(0, rows, n)
+    pos[:, 0] = rng.uniform(-0.99, 0.99, n)
+    pos[:, 1] = (which % 4) * 4 * cell + rng.uniform(0.1, 0.9, n) * cell - 0.5
+    pos[:, 2] = (which // 4) * 4 * cell + rng.uniform(0.1, 0.9, n) * cell - 0.5
+    vel = np.zeros((n, 4), np.float32)
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    OP = orc.params_from(P)
+    ps.run(0.5, 5)
+    for _ in range(5):
+        st.update(0.5, OP)
+    _compare_system(ps, st)
+    ps.close()
+
+
 def test_full_size_steps_match_oracle(gpu_ctx):
     # 1,048,576 spheres r=1/128 on a 256^3 grid: two whole updates, bit-exact
     n, grid, r = 1 << 20, 256, 1.0 / 128.0
