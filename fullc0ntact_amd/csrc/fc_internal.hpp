@@ -305,8 +305,9 @@ struct RowTables {
   uint32_t* startNext = nullptr;  // [rows + 1] row starts of the next order
   uint32_t* chgStart = nullptr;   // [rows x copies + 1] bucket starts of the row changers
   uint32_t* rowList = nullptr;    // [rows] the rows that hold spheres in the next order
-  uint32_t* groupSums = nullptr;  // [3][groups] per 256-row group: spheres, changers, rows
+  uint64_t* groupWords = nullptr; // [groups][3] per 256-row group: spheres, changers, rows (epoch-tagged)
   uint32_t* listCount = nullptr;  // [1] entries of rowList
+  uint32_t* epoch = nullptr;      // [1] steps resorted (tags the group words)
   uint32_t* zeroed = nullptr;     // the counts below, zero between steps
   size_t zeroedWords = 0;
   uint32_t* rowHist = nullptr;    // [rows x copies] next keys per row
@@ -336,7 +337,7 @@ RowHists row_hists(const RowTables& T);
 fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n);
 void free_row_tables(RowTables& T);
 hipError_t launch_row_clear(hipStream_t s, const RowTables& T);
-hipError_t launch_row_scan(hipStream_t s, const RowTables& T);
+hipError_t launch_row_scan(hipStream_t s, const RowTables& T, uint32_t* devFlags);
 // Row starts of an order sorted by another path (the radix sort of a run's
 // first step), read from its sorted hashes.
 hipError_t launch_row_starts(hipStream_t s, const RowTables& T, const uint32_t* sortedHash,

@@ -246,7 +246,7 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
   hipError_t e;
   if (rowMode && keysAtSlots) {
     if (timer) timer->begin(st, "row_scan");
-    e = launch_row_scan(st, sys->rowTables);
+    e = launch_row_scan(st, sys->rowTables, ctx->devFlags);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "row scan");
     if (timer) timer->begin(st, "row_changers");

@@ -9,11 +9,11 @@
 // A grid row (the gridSize.x cells that share y and z) is a contiguous
 // range of that order, so each row is sorted on its own:
 //
-//   k_row_group_sums  per 256-row group, the spheres, row changers and
-//                     occupied rows the key producer counted;
 //   k_row_scan        row starts of the next order (the current ones are
 //                     kept), the changers' bucket starts, and the list of
-//                     the occupied rows;
+//                     the occupied rows, from the per-row counts the key
+//                     producer accumulated (256-row groups pass their
+//                     totals on to the later groups);
 //   k_row_changers    buckets the spheres whose row changes by their new
 //                     row; clears the cells of rows that empty;
 //   k_row_sort        a fixed grid over the occupied rows.  A row's members
@@ -77,10 +77,6 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_wave) {
   return total;
 }
 
-__device__ __forceinline__ uint32_t* group_sums(const RowTables& T, int which) {
-  return T.groupSums + (size_t)which * T.groups;
-}
-
 // The kRowCopies counters of row r (16-byte aligned).
 __device__ __forceinline__ void load_copies(const uint32_t* a, uint32_t r, uint32_t (&c)[kRowCopies]) {
   const uint4* p = reinterpret_cast<const uint4*>(a + (size_t)r * kRowCopies);
@@ -96,51 +92,22 @@ __device__ __forceinline__ void zero_copies(uint32_t* a, uint32_t r) {
 }
 static_assert(kRowCopies == 8, "two 16-byte words of counters per row");
 
-// One block per 256-row group, one row per thread: the group's spheres,
-// changers and occupied rows.
-__global__ __launch_bounds__(kScanThreads) void k_row_group_sums(RowTables T) {
-  __shared__ uint32_t s_wave[3][kScanThreads / 64];
-  const uint32_t r = (blockIdx.x << kRowGroupShift) + threadIdx.x;
-  uint32_t h = 0, c = 0;
-  if (r < T.rows) {
-    uint32_t x[kRowCopies], y[kRowCopies];
-    load_copies(T.rowHist, r, x);
-    load_copies(T.chgHist, r, y);
-#pragma unroll
-    for (uint32_t q = 0; q < kRowCopies; ++q) {
-      h += x[q];
-      c += y[q];
-    }
-  }
-  const uint32_t o = block_sum(h != 0u ? 1u : 0u, s_wave[2]);
-  h = block_sum(h, s_wave[0]);
-  c = block_sum(c, s_wave[1]);
-  if (threadIdx.x == 0) {
-    group_sums(T, 0)[blockIdx.x] = h;
-    group_sums(T, 1)[blockIdx.x] = c;
-    group_sums(T, 2)[blockIdx.x] = o;
-  }
-}
+constexpr uint32_t kScanSpinLimit = 1u << 22;
 
-// One block per 256-row group, one row per thread.  The group's bases are
-// the sums of the earlier groups; each row takes base + its prefix within
-// the group, and each (row, copy) its changers' bucket start.  Each row's
-// current start (startNext, written by the previous step) moves to
-// startCur before it is overwritten; occupied rows are listed in row
-// order; the counters are zeroed for the next producer.
-__global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T) {
+// One block per 256-row group, one row per thread.  The block publishes
+// its group's totals (spheres, changers, occupied rows), each tagged with
+// this step's epoch, then adds the totals of the earlier groups as they
+// appear (every group publishes before it waits, and waits only on earlier
+// ones; the wait is bounded and reported).  Each row then takes its start
+// in the next order (base + prefix within the group) and each (row, copy)
+// its changers' bucket start; the row's current start (startNext, from the
+// previous step) moves to startCur before it is overwritten; occupied
+// rows are listed in row order; the counters are zeroed for the next
+// producer.
+__global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T, uint32_t* devFlags) {
   __shared__ uint32_t s_wave[3][kScanThreads / 64];
   const uint32_t g = blockIdx.x;
-  uint32_t bh = 0, bc = 0, bo = 0;
-  for (uint32_t i = threadIdx.x; i < g; i += kScanThreads) {
-    bh += group_sums(T, 0)[i];
-    bc += group_sums(T, 1)[i];
-    bo += group_sums(T, 2)[i];
-  }
-  bh = block_sum(bh, s_wave[0]);
-  bc = block_sum(bc, s_wave[1]);
-  bo = block_sum(bo, s_wave[2]);
-  __syncthreads();
+  const uint32_t epoch = *T.epoch + 1u;  // the row sort advances *T.epoch after this kernel
   const uint32_t r = (g << kRowGroupShift) + threadIdx.x;
   const bool in = r < T.rows;
   uint32_t x[kRowCopies], y[kRowCopies];
@@ -155,9 +122,47 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T) {
     }
   }
   uint32_t th, tc, to;
-  const uint32_t ph = block_exclusive_scan(h, s_wave[0], &th) + bh;
-  uint32_t pc = block_exclusive_scan(c, s_wave[1], &tc) + bc;
-  const uint32_t po = block_exclusive_scan(h != 0u ? 1u : 0u, s_wave[2], &to) + bo;
+  uint32_t ph = block_exclusive_scan(h, s_wave[0], &th);
+  uint32_t pc = block_exclusive_scan(c, s_wave[1], &tc);
+  uint32_t po = block_exclusive_scan(h != 0u ? 1u : 0u, s_wave[2], &to);
+  uint64_t* pub = T.groupWords + (size_t)g * 3;
+  if (threadIdx.x == 0) {
+    const uint64_t tag = (uint64_t)epoch << 32;
+    __hip_atomic_store(pub + 0, tag | th, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pub + 1, tag | tc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pub + 2, tag | to, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the earlier groups' totals
+  uint32_t bh = 0, bc = 0, bo = 0;
+  bool stalled = false;
+  for (uint32_t i = threadIdx.x; i < g; i += kScanThreads) {
+    const uint64_t* w = T.groupWords + (size_t)i * 3;
+    uint64_t v[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      v[j] = __hip_atomic_load(w + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t spins = 0;
+      while ((uint32_t)(v[j] >> 32) != epoch) {
+        if (++spins > kScanSpinLimit) {
+          stalled = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v[j] = __hip_atomic_load(w + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    bh += (uint32_t)v[0];
+    bc += (uint32_t)v[1];
+    bo += (uint32_t)v[2];
+  }
+  if (stalled) atomicOr(devFlags, DEVERR_SORT_STALLED);
+  __syncthreads();  // s_wave's first use is done
+  bh = block_sum(bh, s_wave[0]);
+  bc = block_sum(bc, s_wave[1]);
+  bo = block_sum(bo, s_wave[2]);
+  ph += bh;
+  pc += bc;
+  po += bo;
   if (in) {
     T.startCur[r] = T.startNext[r];
     T.startNext[r] = ph;
@@ -184,6 +189,15 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T) {
   }
 }
 
+// Start of a run: zero the counters (a run that stopped early may have left
+// counts behind) and move the epoch past any tag a stopped scan published.
+__global__ __launch_bounds__(256) void k_row_reset(RowTables T) {
+  for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < T.zeroedWords;
+       i += (size_t)gridDim.x * 256u)
+    T.zeroed[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *T.epoch += 2u;
+}
+
 // Row starts of a sorted hash array: the first position whose hash lies in
 // row r or beyond (a binary search per row).
 __global__ __launch_bounds__(256) void k_row_starts(RowTables T, const uint32_t* __restrict__ hash,
@@ -206,33 +220,47 @@ __global__ __launch_bounds__(256) void k_row_starts(RowTables T, const uint32_t*
   T.startNext[r] = lo;
 }
 
-// One thread per current sorted entry k (keys[k] is its next key with
-// kMovedBit when the cell changes, hash[k] its current one).  A sphere
+// One thread per current sorted entry k (keys[k] is the next key with
+// kMovedBit when the cell changes, hash[k] the current one).  A sphere
 // whose row changes is listed in its new row's bucket.  A current cell
 // none of whose spheres stays is cleared in the tables (the row sort, which
-// runs next, writes every cell of the next order, this one too if
-// spheres move in).
+// runs next, writes every cell of the next order, this one too if spheres
+// move in): the wave's ballots tell a cell's first lane whether any of the
+// cell's lanes stays; only a cell that goes on past the wave reads on.
 __global__ __launch_bounds__(256) void k_row_changers(RowTables T, const uint32_t* __restrict__ keys,
                                                       const uint32_t* __restrict__ hash,
                                                       uint32_t* __restrict__ cellStart,
                                                       uint32_t* __restrict__ cellEnd, uint32_t n,
                                                       uint32_t numCells, uint32_t* devFlags) {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t raw = keys[k], old = hash[k];
-  const uint32_t key = raw & ~kMovedBit;
-  if (k == 0 || hash[k - 1] != old) {
-    bool stays = (raw & kMovedBit) == 0u;
-    for (uint32_t k2 = k + 1; !stays && k2 < n && hash[k2] == old; ++k2)
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool valid = k < n;
+  const uint32_t raw = valid ? keys[k] : 0u;
+  const uint32_t old = valid ? hash[k] : 0u;
+  const uint32_t before = (lane == 0u && valid && k > 0) ? hash[k - 1] : 0u;
+  const uint32_t after = (lane == 63u && valid && k + 1 < n) ? hash[k + 1] : 0u;
+  uint32_t oldPrev = __shfl_up(old, 1), oldNext = __shfl_down(old, 1);
+  if (lane == 0u) oldPrev = before;
+  if (lane == 63u) oldNext = after;
+  const bool first = valid && (k == 0 || oldPrev != old);
+  const bool last = valid && (k + 1 == n || oldNext != old);
+  const uint64_t lastMask = __ballot(last);
+  const uint64_t staysMask = __ballot(valid && (raw & kMovedBit) == 0u);
+  if (first) {
+    const uint64_t ends = lastMask & lanes_from(lane);
+    const uint32_t t = ends ? lowest_lane(ends) : 63u;
+    bool stays = (staysMask & lanes_from(lane) & ~lanes_from(t + 1u)) != 0ull;
+    // the cell goes on past the wave's last lane
+    for (uint32_t k2 = k + (64u - lane); !stays && ends == 0ull && k2 < n && hash[k2] == old; ++k2)
       stays = (keys[k2] & kMovedBit) == 0u;
     if (!stays && old < numCells) {
       cellStart[old] = CELL_EMPTY;
       cellEnd[old] = CELL_EMPTY;
     }
   }
-  const uint32_t oldRow = old >> T.shift;
-  const uint32_t r = key >> T.shift;
-  if (r == oldRow) return;
+  if (!valid) return;
+  const uint32_t r = (raw & ~kMovedBit) >> T.shift;
+  if (r == (old >> T.shift)) return;
   if (r >= T.rows) {
     atomicOr(devFlags, DEVERR_ROW_COUNT);
     return;
@@ -394,6 +422,7 @@ __global__ __launch_bounds__(kSortThreads) void k_row_sort(RowTables T,
   __shared__ uint32_t s_movers;
   const uint32_t tid = threadIdx.x;
   const uint32_t listed = *T.listCount;
+  if (blockIdx.x == 0 && tid == 0) *T.epoch += 1u;  // the scan of this step is done
   for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
     const uint32_t r = T.rowList[j];
     const uint32_t o0 = T.startNext[r], m = T.startNext[r + 1] - o0;
@@ -544,14 +573,14 @@ fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n) {
     return FC_SUCCESS;
   free_row_tables(T);
   // startCur, startNext [rows + 1]; chgStart [rows x copies + 1]; rowList
-  // [rows]; groupSums [3 groups]; listCount [1]; rowHist, chgHist,
-  // chgCursor [rows x copies] (zeroed); chgList [n]
+  // [rows]; groupWords [3 groups, 64-bit]; listCount, epoch [2]; rowHist,
+  // chgHist, chgCursor [rows x copies] (zeroed); chgList [n]
   // every array starts on a 64-word (256-byte) boundary
   const size_t cells = (size_t)rows * kRowCopies;
   auto up = [](size_t x) { return (x + 63) & ~(size_t)63; };
   const size_t zeroWords = 3 * cells;
-  const size_t words = 2 * up(rows + 1) + up(cells + 1) + up(rows) + up(3 * (size_t)groups) +
-                       up(1) + zeroWords + (n ? n : 1);
+  const size_t words = 2 * up(rows + 1) + up(cells + 1) + up(rows) + up(6 * (size_t)groups) +
+                       up(2) + zeroWords + (n ? n : 1);
   FC_HIP_TRY(hipMalloc(&T.block, words * sizeof(uint32_t)));
   uint32_t* w = T.block;
   T.startCur = w;
@@ -562,10 +591,11 @@ fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n) {
   w += up(cells + 1);
   T.rowList = w;
   w += up(rows);
-  T.groupSums = w;
-  w += up(3 * (size_t)groups);
+  T.groupWords = reinterpret_cast<uint64_t*>(w);
+  w += up(6 * (size_t)groups);
   T.listCount = w;
-  w += up(1);
+  T.epoch = w + 1;
+  w += up(2);
   T.zeroed = w;
   T.zeroedWords = zeroWords;
   T.rowHist = w;
@@ -589,12 +619,13 @@ void free_row_tables(RowTables& T) {
 }
 
 hipError_t launch_row_clear(hipStream_t s, const RowTables& T) {
-  return launch_fill_u32(s, T.zeroed, 0u, T.zeroedWords);
+  const uint32_t blocks = (uint32_t)((T.zeroedWords + 255u) / 256u);
+  hipLaunchKernelGGL(k_row_reset, dim3(blocks < 4096u ? blocks : 4096u), dim3(256), 0, s, T);
+  return hipGetLastError();
 }
 
-hipError_t launch_row_scan(hipStream_t s, const RowTables& T) {
-  hipLaunchKernelGGL(k_row_group_sums, dim3(T.groups), dim3(kScanThreads), 0, s, T);
-  hipLaunchKernelGGL(k_row_scan, dim3(T.groups), dim3(kScanThreads), 0, s, T);
+hipError_t launch_row_scan(hipStream_t s, const RowTables& T, uint32_t* devFlags) {
+  hipLaunchKernelGGL(k_row_scan, dim3(T.groups), dim3(kScanThreads), 0, s, T, devFlags);
   return hipGetLastError();
 }
 
