@@ -144,6 +144,10 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
         return (12 if p == 0 else 16) * n + ((72 * n + (8 * n if run else 0)) if last else 0)
     if name == "integrate_hash":  # pos/vel read+write 64 B, key/value 8 B
         return 72 * n
+    if name == "row_changers":  # next keys and current hashes 8 B (the few changers' bucket
+        return 8 * n  # entries and the emptied cells' table entries on top)
+    if name == "row_sort":  # keys and ids 8 B, pos/vel gathered 32 B; hash, id 8 B, sortedPos/
+        return 88 * n + 8 * c_occ  # sortedVel 32 B and fixed-point coordinates 8 B written; tables
     return None
 
 
