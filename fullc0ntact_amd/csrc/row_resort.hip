@@ -35,19 +35,24 @@ namespace fc {
 namespace {
 
 constexpr int kScanThreads = 256;
-constexpr int kSortThreads = 256;
-constexpr uint32_t kRowCap = 1536;     // members of a row held in LDS (18 KB: 8 blocks a CU)
-constexpr uint32_t kMoverCap = 512;    // movers ranked by counting
+// The row sort: one row per 128-thread block, 9 KB of LDS, so that 16
+// blocks fit a CU and the ~3800 occupied rows of a 1M-sphere bed are all in
+// flight at once (each row is a chain of dependent loads).
+constexpr int kSortThreads = 128;
+constexpr uint32_t kRowCap = 768;      // members of a row held in LDS
+constexpr uint32_t kMoverCap = 256;    // movers ranked by counting
 constexpr uint32_t kPerThread = kRowCap / kSortThreads;
 constexpr uint32_t kMoversPerThread = kMoverCap / kSortThreads;
-constexpr uint32_t kSortBlocks = 4096; // grid of the row sort
+constexpr uint32_t kSortBlocks = 8192; // grid of the row sort
 static_assert(kRowCap % kSortThreads == 0 && kMoverCap % kSortThreads == 0, "whole per-thread shares");
 constexpr uint64_t kNoMember = ~0ull;
 static_assert(kScanThreads == (1 << kRowGroupShift), "one row per scan thread");
 
-// Exclusive scan of one value per thread over a 256-thread block; returns
-// the thread's prefix and writes the block total to *total.  s_wave holds
-// 4 words; the caller separates two uses of one s_wave by a barrier.
+// Exclusive scan of one value per thread over a kThreads-thread block;
+// returns the thread's prefix and writes the block total to *total.  s_wave
+// holds kThreads/64 words; the caller separates two uses of one s_wave by a
+// barrier.
+template <int kThreads = kScanThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave,
                                                          uint32_t* total) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -61,7 +66,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   __syncthreads();
   uint32_t before = 0, all = 0;
 #pragma unroll
-  for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+  for (uint32_t w = 0; w < kThreads / 64; ++w) {
     const uint32_t t = s_wave[w];
     if (w < wave) before += t;
     all += t;
@@ -71,9 +76,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 // Sum of one value per thread over the block (every thread gets it).
+template <int kThreads = kScanThreads>
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_wave) {
   uint32_t total;
-  block_exclusive_scan(v, s_wave, &total);
+  block_exclusive_scan<kThreads>(v, s_wave, &total);
   return total;
 }
 
@@ -404,7 +410,8 @@ __device__ void rank_by_counting(const RowTables& T, const uint32_t* keys,
         atomicOr(devFlags, DEVERR_ROW_COUNT);
     }
   }
-  if (block_sum(members, s_wave) != m && tid == 0) atomicOr(devFlags, DEVERR_ROW_COUNT);
+  if (block_sum<kSortThreads>(members, s_wave) != m && tid == 0)
+    atomicOr(devFlags, DEVERR_ROW_COUNT);
 }
 
 // The next keys carry kMovedBit; the sorted hashes are written to O.hash
@@ -418,7 +425,7 @@ __global__ __launch_bounds__(kSortThreads) void k_row_sort(RowTables T,
   // after the merge the row's members in order from the front
   __shared__ uint64_t s_key[kRowCap];
   __shared__ uint32_t s_slot[kRowCap];
-  __shared__ uint32_t s_wave[kScanThreads / 64];
+  __shared__ uint32_t s_wave[kSortThreads / 64];
   __shared__ uint32_t s_movers;
   const uint32_t tid = threadIdx.x;
   const uint32_t listed = *T.listCount;
@@ -447,7 +454,7 @@ __global__ __launch_bounds__(kSortThreads) void k_row_sort(RowTables T,
       mine += (((raw & ~kMovedBit) >> T.shift) == r && (raw & kMovedBit) == 0u) ? 1u : 0u;
     }
     uint32_t stayers;  // the same in every thread
-    uint32_t at = block_exclusive_scan(mine, s_wave, &stayers);
+    uint32_t at = block_exclusive_scan<kSortThreads>(mine, s_wave, &stayers);
 #pragma unroll 4
     for (uint32_t u = u0; u < u1; ++u) {
       const uint32_t k = a0 + u;
