@@ -409,6 +409,31 @@ def test_run_with_cell_mixing_matches_oracle(gpu_ctx, grid):
     ps.close()
 
 
+def test_run_with_many_occupied_rows_matches_oracle(gpu_ctx):
+    # 65,536 spheres spread over the 16,384 rows of a 128^3 grid: more
+    # occupied rows than the row sort has blocks, so its blocks take
+    # several rows each.
+    n, grid, r = 65536, 128, 1.0 / 128.0
+    print(f"[rows] n={n} grid={grid}", flush=True)
+    from fullc0ntact_amd import ParticleSystem
+    ps = ParticleSystem(n, (grid, grid, grid))
+    P = sim_params(n, grid, r)
+    ps.m_params = P
+    pos, vel = orc.reset(n, 0, 5, r, 0.0, 0.01)
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    OP = orc.params_from(P)
+    ps.run(0.5, 12)
+    for _ in range(12):
+        st.update(0.5, OP)
+    _compare_system(ps, st)
+    rows_used = len(np.unique(st.hash // grid))
+    print(f"[rows] occupied rows {rows_used}", flush=True)
+    assert rows_used > 8192
+    ps.close()
+
+
 @pytest.mark.parametrize("rows", [1, 8, 16])
 def test_run_with_crowded_rows_matches_oracle(gpu_ctx, rows):
     # 20,000 spheres packed into `rows` grid rows of a 32^3 grid (x spread
