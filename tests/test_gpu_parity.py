@@ -409,6 +409,33 @@ def test_run_with_cell_mixing_matches_oracle(gpu_ctx, grid):
     ps.close()
 
 
+def test_runs_and_updates_interleaved_match_oracle(gpu_ctx):
+    # Runs of 2, 3 and 9 steps (a 2-step run is the head, one radix-sorted
+    # step and one row-resorted last step) between single updates, on one
+    # system of 40,000 spheres.
+    n, grid, r = 40_000, 64, 1.0 / 64.0
+    print(f"[interleave] n={n}", flush=True)
+    from fullc0ntact_amd import ParticleSystem
+    ps = ParticleSystem(n, (grid, grid, grid))
+    P = sim_params(n, grid, r)
+    ps.m_params = P
+    pos, vel = orc.reset(n, 0, 21, r, 0.0, 0.02)
+    ps.setArray(ps.POSITION, pos, 0, n)
+    ps.setArray(ps.VELOCITY, vel, 0, n)
+    st = orc.State(pos, vel, P.numCells)
+    OP = orc.params_from(P)
+    for kind, steps in (("run", 2), ("update", 1), ("run", 3), ("update", 2), ("run", 9)):
+        if kind == "run":
+            ps.run(0.5, steps)
+        else:
+            for _ in range(steps):
+                ps.update(0.5)
+        for _ in range(steps):
+            st.update(0.5, OP)
+        _compare_system(ps, st)
+    ps.close()
+
+
 def test_run_with_many_occupied_rows_matches_oracle(gpu_ctx):
     # 65,536 spheres spread over the 16,384 rows of a 128^3 grid: more
     # occupied rows than the row sort has blocks, so its blocks take
