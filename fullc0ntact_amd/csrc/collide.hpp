@@ -52,11 +52,16 @@ constexpr int kListCap = FC_LIST_CAP;
 // iteration of its phase B (loads issued together).  The kernel is bound by
 // the latency of its dependent neighbour loads more than by issue: on the
 // settled 1M-sphere bed, batches of (2, 1) measured 136 us per launch,
-// (4, 2) 122 us and (4, 4) 113 us; (8, 4) needs 97 VGPRs and gained
-// nothing.  Raising occupancy instead (list capacity 20 or 16 with 6-8
-// waves per SIMD) spilled registers and was slower.
+// (4, 2) 122 us and (4, 4) 113 us; (8, 4) needed 97 VGPRs and gained
+// nothing while phase A loaded one candidate per lane request.  Raising
+// occupancy instead (list capacity 20 or 16 with 6-8 waves per SIMD)
+// spilled registers and was slower.  With two candidates per 16-byte load
+// (the fused kernel of a later build): (4, 4) 98.3 us, (8, 4) 95.5; with
+// the cheaper squared distance of quant_d2 as well, (8, 4) 93.6-94.4,
+// (12, 4) 95.0-95.1, (16, 4) 109.7 (registers spilled), (8, 2) 96.5-97.0,
+// (8, 6) 93.7-93.9.
 #ifndef FC_ROWS_BATCH
-#define FC_ROWS_BATCH 4
+#define FC_ROWS_BATCH 8
 #endif
 #ifndef FC_PAIR_BATCH
 #define FC_PAIR_BATCH 4
@@ -224,16 +229,18 @@ __device__ __forceinline__ T load_off(const T* __restrict__ base, uint32_t byteO
 }
 
 // Squared distance of two packed fixed-point coordinate triples
-// (quantize_pos): the per-axis differences in 16-bit lanes, then two packed
-// dot products.  Exact whenever every difference fits int16 (the sum of
-// three squares of at most 2^15 stays below 2^32); a larger difference
-// wraps and may read short, which only keeps a far pair as a candidate.
+// (quantize_pos): the x and y differences in 16-bit lanes and one packed
+// dot product, added to the square of the z difference (exact: the z
+// coordinates are the low halves, so the difference fits 17 bits and its
+// square 32), all modulo 2^32.  Exact whenever every difference fits int16
+// (the sum of three squares of at most 2^15 stays below 2^32); a larger
+// difference wraps and may read short, which only keeps a far pair as a
+// candidate.
 __device__ __forceinline__ uint32_t quant_d2(uint2 a, uint2 b) {
   typedef short s16x2 __attribute__((ext_vector_type(2)));
   const s16x2 dxy = __builtin_bit_cast(s16x2, a.x) - __builtin_bit_cast(s16x2, b.x);
-  const s16x2 dz = __builtin_bit_cast(s16x2, a.y) - __builtin_bit_cast(s16x2, b.y);
-  return (uint32_t)__builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, 0, false),
-                                          false);
+  const int dz = (int)a.y - (int)b.y;
+  return (uint32_t)__builtin_amdgcn_sdot2(dxy, dxy, __mul24(dz, dz), false);
 }
 
 // Row-range variant of collide_one_v3 for grids with every dimension >= 4,
@@ -349,34 +356,50 @@ __device__ __forceinline__ v3f collide_one_rows(const fc_sim_params& P, v3f invC
 
   // ---- phase A: candidates within the contact pre-filter ----
   // The three rows of a z-plane are walked as one flattened sequence of
-  // slots, kRowsBatch per iteration with their loads in flight together.
-  // Slot t of the plane is sphere t + q(t), q stepping at the row
-  // boundaries; the list keeps the candidates' sorted indices.  A slot
-  // past the plane's end reads at most kRowsBatch - 1 entries past the last
-  // row's range (inside the array's padding) and its test is masked off.
-  // Only the centre plane holds the sphere itself.
+  // slots in which each row's length is rounded up to even, so that slots
+  // 2p and 2p + 1 are adjacent spheres of one row and one 16-byte load
+  // brings both candidates' coordinates; kRowsBatch slots per iteration
+  // with their loads in flight together.  Slot t of the plane is sphere
+  // t + q(t), q stepping at the (even) row boundaries; the list keeps the
+  // candidates' sorted indices.  The extra slot of an odd row (the sphere
+  // after the row) and the slots past the plane's end are masked off; the
+  // latter read at most kRowsBatch - 1 entries past the last row's range
+  // (inside the array's padding).  Only the centre plane holds the sphere
+  // itself.
+  typedef uint32_t u32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+  static_assert(kRowsBatch % 2 == 0, "whole slot pairs per iteration");
   const uint2 me = sortedQ[k];
   uint32_t cnt = 0;
 #pragma unroll
   for (int pl = 0; pl < 3; ++pl) {
-    const uint32_t b1 = len[3 * pl], b2 = b1 + len[3 * pl + 1], tot = b2 + len[3 * pl + 2];
+    const uint32_t l0 = len[3 * pl], l1 = len[3 * pl + 1], l2 = len[3 * pl + 2];
+    const uint32_t b1 = (l0 + 1u) & ~1u, b2 = b1 + ((l1 + 1u) & ~1u);
+    const uint32_t tot = b2 + ((l2 + 1u) & ~1u);
+    const uint32_t e1 = b1 + l1, e2 = b2 + l2;  // ends of the real slots (row 0: l0)
     const uint32_t q0 = lo[3 * pl], q1 = lo[3 * pl + 1] - b1,
                    q2 = lo[3 * pl + 2] - b2;  // modulo 2^32
 #pragma unroll 1
     for (uint32_t t = 0; t < tot; t += kRowsBatch) {
-      uint32_t idx[kRowsBatch];
-      uint2 qq[kRowsBatch];
+      uint32_t idx[kRowsBatch / 2], end[kRowsBatch / 2];
+      u32x4_a8 qq[kRowsBatch / 2];
 #pragma unroll
-      for (int u = 0; u < kRowsBatch; ++u) {
-        const uint32_t tu = t + (uint32_t)u;
-        idx[u] = tu + (tu < b1 ? q0 : (tu < b2 ? q1 : q2));
-        qq[u] = load_off(sortedQ, idx[u] << 3);
+      for (int h = 0; h < kRowsBatch / 2; ++h) {
+        const uint32_t th = t + 2u * (uint32_t)h;
+        const bool r0 = th < b1, r1 = th < b2;
+        idx[h] = th + (r0 ? q0 : (r1 ? q1 : q2));
+        end[h] = r0 ? l0 : (r1 ? e1 : e2);
+        qq[h] = *reinterpret_cast<const u32x4_a8*>(reinterpret_cast<const char*>(sortedQ) +
+                                                   (idx[h] << 3));
       }
 #pragma unroll
       for (int u = 0; u < kRowsBatch; ++u) {
-        const uint32_t d2 = quant_d2(qq[u], me);
-        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = idx[u];
-        const bool in = (u == 0 || t + (uint32_t)u < tot) && (pl != 1 || idx[u] != k);
+        const int h = u >> 1;
+        const uint32_t j = idx[h] + (uint32_t)(u & 1);
+        const uint2 c = (u & 1) ? make_uint2(qq[h].z, qq[h].w) : make_uint2(qq[h].x, qq[h].y);
+        const uint32_t d2 = quant_d2(c, me);
+        s_list[cnt < (uint32_t)kListCap ? cnt : (uint32_t)kListCap][threadIdx.x] = j;
+        // (the first slot of the first pair is always a real one)
+        const bool in = (u == 0 || t + (uint32_t)u < end[h]) && (pl != 1 || j != k);
         cnt += (d2 < qThreshold && in) ? 1u : 0u;
       }
     }

@@ -4,7 +4,8 @@ must keep every pair that the reference counts as touching.
 The device arithmetic is emulated bit for bit on the host: the coordinate
 u = rint((p - lo) * scale) in float32 (separate rounding of the difference
 and the product, as compiled without contraction), clamped to [0, 65535];
-the differences in wrapping 16-bit lanes; the squares summed exactly.  The
+the x and y differences in wrapping 16-bit lanes, the z difference exact;
+the squares summed modulo 2^32.  The
 reference's contact test is float32 `sqrtf(dot(rel, rel)) < rA + rB`.  Pairs
 are placed at the contact distance to the last ulp, in random directions
 (axis-aligned and diagonal ones included), all over the frame and beyond
@@ -34,8 +35,9 @@ def quantize(p, lo, scale):
 
 
 def quant_d2(ua, ub):
-    d = (ua - ub + 32768) % 65536 - 32768  # int16 lanes
-    return (d * d).sum(axis=1) % (1 << 32)
+    dxy = (ua[:, :2] - ub[:, :2] + 32768) % 65536 - 32768  # int16 lanes
+    dz = ua[:, 2] - ub[:, 2]                                # exact
+    return ((dxy * dxy).sum(axis=1) + dz * dz) % (1 << 32)
 
 
 def ref_touch(pa, pb, cd):
