@@ -74,6 +74,24 @@ __global__ __launch_bounds__(kBlock) void k_calc_hash(fc_sim_params P, uint32_t*
   index[i] = i;
 }
 
+// The state the row-range collide reads, built from the reference-style
+// sorted arrays of fc_collide / fc_eval_forces, which carry neither: each
+// sorted sphere's velocity with the bits of its cell hash in w (the hash of
+// its position, as calcHash computes it, so the hash it was sorted by when
+// the arrays come from reorderDataAndFindCellStart of those hashes), and its
+// 16-bit coordinates in `frame`.
+__global__ __launch_bounds__(kBlock) void k_rows_state(fc_sim_params P, QuantFrame frame,
+                                                       const float4* __restrict__ sortedPos,
+                                                       const float4* __restrict__ sortedVel,
+                                                       uint32_t n, float4* __restrict__ velOut,
+                                                       uint2* __restrict__ qOut) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = sortedPos[i], v = sortedVel[i];
+  velOut[i] = make_float4(v.x, v.y, v.z, __uint_as_float(hash_of(P, p.x, p.y, p.z)));
+  qOut[i] = quantize_pos(frame, p.x, p.y, p.z);
+}
+
 // integrate + calcHash in one pass (the update order runs them back to back
 // on the same arrays, so fusing them changes no value), with two more jobs
 // folded into the same read of the particles:
@@ -473,6 +491,15 @@ hipError_t launch_integrate(hipStream_t s, const fc_sim_params& P, float4* pos, 
   return hipGetLastError();
 }
 
+hipError_t launch_rows_state(hipStream_t s, const fc_sim_params& P, const QuantFrame& frame,
+                             const float4* sortedPos, const float4* sortedVel, uint32_t n,
+                             float4* velOut, uint2* qOut) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_state, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, P, frame,
+                     sortedPos, sortedVel, n, velOut, qOut);
+  return hipGetLastError();
+}
+
 hipError_t launch_calc_hash(hipStream_t s, const fc_sim_params& P, uint32_t* hash,
                             uint32_t* index, const float4* pos, uint32_t n) {
   if (n == 0) return hipSuccess;
@@ -526,10 +553,11 @@ float contact_d2_threshold(float R) {
   return t;
 }
 
-// The row-range walk needs arrays gathered by the sort (the sorted hash in
-// sortedVel.w, the fixed-point coordinates, kSortedPosPad entries of
-// padding: a system's own arrays, GatheredState), 27 distinct neighbour
-// cells (every grid dimension >= 4 here) and cell byte offsets below 2^32.
+// The row-range walk needs the sorted hash in sortedVel.w and the
+// fixed-point coordinates with kSortedPosPad entries of padding
+// (GatheredState: a system's arrays gathered by its sort, or the ones
+// k_rows_state builds), 27 distinct neighbour cells (every grid dimension
+// >= 4 here) and cell byte offsets below 2^32.
 static bool rows_walk_ok(const fc_sim_params& P) {
   return P.gridSize.x >= 4 && P.gridSize.y >= 4 && P.gridSize.z >= 4 &&
          P.numCells <= (1u << 30);
@@ -582,7 +610,8 @@ static void launch_v3(hipStream_t s, const GatheredState* gs, dim3 g, dim3 b,
                       uint32_t* devFlags, uint32_t* slotOf, uint32_t* idOfSlot,
                       float* forceOut = nullptr, bool slotKeys = false,
                       const RowHists* rows = nullptr) {
-  if (gs != nullptr && rows_walk_ok(P) && n <= kSplitMaxSpheres) {
+  // (the split kernel reads the reference layout only: no gathered arrays)
+  if (rows_walk_ok(P) && n <= kSplitMaxSpheres) {
     const uint32_t waves = (n + kSplitSpheresPerWave - 1) / kSplitSpheresPerWave;
     hipLaunchKernelGGL((k_collide_split<kFuse, kPoly>), dim3((waves + 3) / 4), b, 0, s, P, B, d2,
                        newVel, sortedPos, sortedVel, index, cellStart, cellEnd, n, pos, dt,
@@ -635,6 +664,10 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
 // map, for the radix sort).
 bool collide_slot_keys(const fc_sim_params& P, uint32_t n) {
   return n > kSplitMaxSpheres && rows_walk_ok(P) && row_resort_grid_ok(P);
+}
+
+bool collide_rows_state_pays(const fc_sim_params& P, uint32_t n) {
+  return rows_walk_ok(P) && n > kSplitMaxSpheres && n < (1u << 27);
 }
 
 bool collide_can_fuse(const fc_sim_params& P, uint32_t n) {

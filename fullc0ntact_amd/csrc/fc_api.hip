@@ -220,6 +220,53 @@ using namespace fc;
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+
+// The reference-style collide (collideD; with `force`, evalForces_launcher)
+// on the caller's sorted arrays.  Large systems take the row-range kernel
+// on the state launch_rows_state builds into the context's buffers (the
+// caller's arrays carry neither the sorted hash nor the fixed-point
+// coordinates), small ones the row-split kernel on the caller's arrays,
+// grids that neither walk takes the cell-table walk.  The buffers grow on
+// the first call with more spheres than they hold (an allocation, so not
+// inside a graph capture).
+fc_status collide_reference_arrays(fc_context* ctx, float* newVel, float* force,
+                                   const float* sortedPos, const float* sortedVel,
+                                   const uint32_t* index, const uint32_t* cellStart,
+                                   const uint32_t* cellEnd, uint32_t n) {
+  const fc_sim_params& P = ctx->params;
+  const float4* vel = reinterpret_cast<const float4*>(sortedVel);
+  GatheredState gs;
+  const GatheredState* gathered = nullptr;
+  if (collide_rows_state_pays(P, n)) {
+    if (ctx->rowsCapacity < n) {
+      FC_HIP_TRY(hipStreamSynchronize(ctx->stream));  // earlier calls may still read them
+      hipFree(ctx->rowsVel);
+      hipFree(ctx->rowsQ);
+      ctx->rowsVel = nullptr;
+      ctx->rowsQ = nullptr;
+      ctx->rowsCapacity = 0;
+      FC_HIP_TRY(hipMalloc(&ctx->rowsVel, (size_t)n * sizeof(float4)));
+      FC_HIP_TRY(hipMalloc(&ctx->rowsQ, ((size_t)n + kSortedPosPad) * sizeof(uint2)));
+      ctx->rowsCapacity = n;
+    }
+    gs.sortedQ = ctx->rowsQ;
+    gs.frame = quant_frame_for(false);  // integrate's box walls
+    FC_HIP_TRY(launch_rows_state(ctx->stream, P, gs.frame, (const float4*)sortedPos, vel, n,
+                                 ctx->rowsVel, ctx->rowsQ));
+    vel = ctx->rowsVel;
+    gathered = &gs;
+  }
+  const hipError_t e =
+      launch_collide(ctx->stream, P, false, ctx->devFlags, (float4*)newVel,
+                     (const float4*)sortedPos, vel, index, cellStart, cellEnd, n, nullptr,
+                     gathered, force);
+  if (e != hipSuccess) return hip_error(e, force ? "evalForces" : "collide");
+  return FC_SUCCESS;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* fc_last_error(void) { return g_lastError.c_str(); }
@@ -287,6 +334,8 @@ fc_status fc_context_destroy(fc_context* ctx) {
   hipFree(s.counters);
   hipFree(ctx->devFlags);
   hipFree(ctx->statsScratch);
+  hipFree(ctx->rowsVel);
+  hipFree(ctx->rowsQ);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return FC_SUCCESS;
@@ -422,12 +471,10 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
     return set_error(FC_ERROR_INVALID_VALUE, "numCells differs from the context parameters");
   StageTimer* t = stage_timer(ctx);
   if (t) t->begin(ctx->stream, "drop_in_collide");
-  hipError_t e = launch_collide(ctx->stream, ctx->params, false, ctx->devFlags, (float4*)newVel,
-                                (const float4*)sortedPos, (const float4*)sortedVel, index,
-                                cellStart, cellEnd, numParticles);
+  const fc_status st = collide_reference_arrays(ctx, newVel, nullptr, sortedPos, sortedVel, index,
+                                                cellStart, cellEnd, numParticles);
   if (t) t->end(ctx->stream);
-  if (e != hipSuccess) return hip_error(e, "collide");
-  return FC_SUCCESS;
+  return st;
 }
 
 fc_status fc_eval_forces(fc_context* ctx, float* newVel, float* force, const float* sortedPos,
@@ -439,11 +486,8 @@ fc_status fc_eval_forces(fc_context* ctx, float* newVel, float* force, const flo
     return set_error(FC_ERROR_INVALID_VALUE, "NULL array");
   if (numCells != ctx->params.numCells)
     return set_error(FC_ERROR_INVALID_VALUE, "numCells differs from the context parameters");
-  hipError_t e = launch_collide(ctx->stream, ctx->params, false, ctx->devFlags, (float4*)newVel,
-                                (const float4*)sortedPos, (const float4*)sortedVel, index,
-                                cellStart, cellEnd, numParticles, nullptr, nullptr, force);
-  if (e != hipSuccess) return hip_error(e, "evalForces");
-  return FC_SUCCESS;
+  return collide_reference_arrays(ctx, newVel, force, sortedPos, sortedVel, index, cellStart,
+                                  cellEnd, numParticles);
 }
 
 fc_status fc_context_enable_timing(fc_context* ctx, int enable) {

@@ -44,8 +44,9 @@ enum : uint32_t {
 };
 
 constexpr uint32_t CELL_EMPTY = 0xffffffffu;
-// Entries allocated past the last sphere in a system's sortedPos and
-// sortedQ (read, and discarded, by the row-range collide's batched loads).
+// Entries allocated past the last sphere in the fixed-point coordinates
+// sortedQ (read, and discarded, by the row-range collide's batched loads;
+// a system's sortedPos is padded alike).
 constexpr uint32_t kSortedPosPad = 8;
 
 // ---------------------------------------------------------------------------
@@ -108,6 +109,16 @@ struct QuantFrame {
   float lo[3] = {0.0f, 0.0f, 0.0f};
   float scale = 0.0f;
 };
+
+// The frame of the region integrate keeps spheres in: the box [-1, 1]^3
+// (box walls), or [-1, 1]^2 x [-1, 10] with a distance-map boundary, one
+// scale for all three axes.
+inline QuantFrame quant_frame_for(bool distanceMap) {
+  QuantFrame f;
+  f.lo[0] = f.lo[1] = f.lo[2] = -1.0f;
+  f.scale = 65535.0f / (distanceMap ? 11.0f : 2.0f);
+  return f;
+}
 
 // The packed coordinates of one sphere: (x | y << 16, z).  A NaN
 // coordinate maps to 0.
@@ -227,10 +238,11 @@ hipError_t launch_reorder(hipStream_t s, uint32_t* cellStart, uint32_t* cellEnd,
 // posOut (optional): also scatter the sorted positions to id order.
 // forceOut (optional): evalForces -- no cursor sphere, and the 27 cells'
 // force stored as float3 at the original index.
-// Gathered state (the row-range walk): sortedPos/sortedVel/sortedQ were
-// written by a sort's SortGather into a system's arrays (sortedVel.w holds
-// the sorted hash, sortedPos and sortedQ carry kSortedPosPad entries of
-// padding, sortedQ's frame is `frame`).
+// Gathered state (the row-range walk): sortedVel.w holds the sorted hash
+// and sortedQ (kSortedPosPad entries of padding) the fixed-point
+// coordinates in `frame`: a system's arrays written by its sort's
+// SortGather, or the ones launch_rows_state builds for the reference-style
+// collide.
 struct GatheredState {
   const uint2* sortedQ = nullptr;
   QuantFrame frame;
@@ -240,6 +252,18 @@ hipError_t launch_collide(hipStream_t s, const fc_sim_params& P, bool poly, uint
                           const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, float4* posOut = nullptr,
                           const GatheredState* gathered = nullptr, float* forceOut = nullptr);
+
+// For the reference-style fc_collide / fc_eval_forces: builds the state
+// the row-range collide reads (velOut = sortedVel with the bits of each
+// position's cell hash in w, qOut the 16-bit coordinates in frame, n +
+// kSortedPosPad entries) from the reference's sorted arrays.
+hipError_t launch_rows_state(hipStream_t s, const fc_sim_params& P, const QuantFrame& frame,
+                             const float4* sortedPos, const float4* sortedVel, uint32_t n,
+                             float4* velOut, uint2* qOut);
+// Whether the reference-style collide takes the row-range kernel through
+// launch_rows_state (large systems; small ones take the row-split kernel
+// on the reference's arrays as they are).
+bool collide_rows_state_pays(const fc_sim_params& P, uint32_t n);
 
 float contact_d2_threshold(float R);
 // Exact reciprocals of the cell sizes when all three are powers of two (the
@@ -291,6 +315,11 @@ struct fc_context {
   uint32_t* devFlags = nullptr;   // device error word
   fc::StageTimer timer;
   double* statsScratch = nullptr; // FP64 reduction partials + histogram bins
+  // State built for the row-range walk by the reference-style collide
+  // (launch_rows_state), grown to the largest system it has seen.
+  float4* rowsVel = nullptr;
+  uint2* rowsQ = nullptr;
+  uint32_t rowsCapacity = 0;
 };
 
 namespace fc {

@@ -379,11 +379,7 @@ using namespace fc;
 // walls; with a distance map, x and y within the unit cylinder and z in
 // [-1, 10].  One scale for all axes keeps fixed-point distances isotropic.
 fc::QuantFrame fc_system::quant_frame() const {
-  fc::QuantFrame f;
-  const bool map = boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr;
-  f.lo[0] = f.lo[1] = f.lo[2] = -1.0f;
-  f.scale = 65535.0f / (map ? 11.0f : 2.0f);
-  return f;
+  return fc::quant_frame_for(boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr);
 }
 
 fc::BoundaryArg fc_system::boundary_arg() const {

@@ -182,7 +182,13 @@ fc_status fc_reorder_data_and_find_cell_start(fc_context* ctx, uint32_t* cellSta
 
 /* collide (particleSystem.cuh:52-59; collideD, particles_kernel.cu:256-293):
  * 27-cell spring-dashpot-shear force plus the collider sphere, written as
- * newVel[index[k]] = (vel + force, 0). */
+ * newVel[index[k]] = (vel + force, 0).  Above 16384 particles the context
+ * first builds, in buffers of its own, each sorted particle's velocity with
+ * its cell hash and its 16-bit coordinates (the arrays are assumed to come
+ * from fc_reorder_data_and_find_cell_start of the hashes fc_calc_hash gives
+ * for these positions, as in the reference's update); the buffers grow on
+ * the first call with more particles than they hold, which allocates, so
+ * such a call cannot be captured in a graph. */
 fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, const float* sortedVel,
                      const uint32_t* gridParticleIndex, const uint32_t* cellStart,
                      const uint32_t* cellEnd, uint32_t numParticles, uint32_t numCells);
@@ -194,7 +200,8 @@ fc_status fc_collide(fc_context* ctx, float* newVel, const float* sortedPos, con
  * force[3 * index[k] + 0..2] = force (float3 array, 12-byte stride, the
  * ParticleWorld::force_ layout).  The reference reads the grid and the
  * particle arrays from its HashGrid/ParticleWorld structs; here they are
- * passed as device pointers, with the parameters of the context. */
+ * passed as device pointers, with the parameters of the context.  Large
+ * systems use the context's buffers as fc_collide does. */
 fc_status fc_eval_forces(fc_context* ctx, float* newVel, float* force, const float* sortedPos,
                          const float* sortedVel, const uint32_t* gridParticleIndex,
                          const uint32_t* cellStart, const uint32_t* cellEnd,

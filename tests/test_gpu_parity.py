@@ -120,10 +120,13 @@ class Stages:
 
 
 CASES = {
-    # 4096 spheres r=1/64, grid 64^3
+    # 4096 spheres r=1/64, grid 64^3 (collide: the row-split kernel)
     "cfg0": dict(n=4096, grid=64, radius=1.0 / 64.0),
     # ragged size, not a multiple of any block or tile
     "ragged": dict(n=5003, grid=64, radius=1.0 / 64.0),
+    # past the row-split kernel's range: the collide takes the row-range
+    # kernel on the state it builds from the reference arrays
+    "large": dict(n=50_021, grid=128, radius=1.0 / 128.0),
 }
 
 
@@ -660,11 +663,18 @@ def test_system_collide_contact_distance_edges(gpu_ctx, poly):
     ps.close()
 
 
-def test_collide_list_overflow_path(gpu_ctx):
+@pytest.mark.parametrize("grid", ["16^3", "16x16x2"])
+def test_collide_list_overflow_path(gpu_ctx, grid):
     # 40 overlapping spheres in one cell: every particle has more near
-    # neighbours than the per-lane contact list holds.
-    n, grid, r = 40, 16, 1.0 / 16.0
-    P = sim_params(n, grid, r)
+    # neighbours than the per-lane contact list holds.  On 16^3 the
+    # reference-style collide takes the row-split kernel; a grid dimension
+    # below 4 sends it to the cell-table walk, through its list overflow.
+    n, r = 40, 1.0 / 16.0
+    print(f"[overflow] grid={grid}", flush=True)
+    if grid == "16^3":
+        P = sim_params(n, 16, r)
+    else:
+        P = sim_params(n, 16, r, gridSize=(16, 16, 2), numCells=16 * 16 * 2)
     OP = orc.params_from(P)
     S = Stages(gpu_ctx, P)
     rng = np.random.default_rng(5)
