@@ -448,7 +448,7 @@ def batch_leg(steps: int) -> dict:
     log(f"[bench] batch: {BATCH_SCENES} scenes, {b.n} spheres; autotuning")
     b.run(DT, 2)
     best, ms = b.autotune(DT, 3)
-    b.run(DT, 3)  # graph capture with the chosen shape
+    b.run(DT, 10)  # graph capture of the run's steps with the chosen shape
     ctx.sync()
     region_resume("batch")
     t0 = time.perf_counter()

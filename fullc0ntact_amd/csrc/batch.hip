@@ -51,6 +51,8 @@ struct fc_batch {
   uint32_t* hash = nullptr;
   uint32_t* index = nullptr;
   uint32_t* keys = nullptr;
+  uint32_t* slotOf = nullptr;    // multi-step runs: slot of each particle id (sort values)
+  uint32_t* idOfSlot = nullptr;  // multi-step runs: particle id held by each slot
   uint32_t* cellStart = nullptr;
   uint32_t* cellEnd = nullptr;
   double* stats = nullptr;  // S x 4
@@ -58,6 +60,10 @@ struct fc_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   float graphDt = 0.0f;
+  // kBatchRunSteps steady-state steps of a multi-step run
+  hipGraph_t runGraph = nullptr;
+  hipGraphExec_t runGraphExec = nullptr;
+  float runGraphDt = 0.0f;
   fc_batch_tuning tuning{4, 256, 0.0f};
 };
 
@@ -151,10 +157,36 @@ __global__ __launch_bounds__(BS) void k_batch_collide(
   newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
 }
 
+// The scene of the first lane of `pending`, wave-uniform (so that what is
+// indexed by it is read with scalar loads into scalar registers).
+__device__ __forceinline__ uint32_t lead_scene(uint64_t pending, uint32_t s) {
+  const int first = __ffsll((unsigned long long)pending) - 1;
+  return (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)s, first));
+}
+
+// The per-scene blocks as read-only (constant address space) memory, which
+// the kernels never write: a load at a wave-uniform index then goes to
+// scalar registers.
+template <typename T>
+__device__ __forceinline__ T scene_block(const T* p, uint32_t s) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+  const ConstWords w = (ConstWords)(const void*)(p + s);
+  uint32_t words[sizeof(T) / 4];
+#pragma unroll
+  for (size_t i = 0; i < sizeof(T) / 4; ++i) words[i] = w[i];
+  T out;
+  std::memcpy(&out, words, sizeof(T));
+  return out;
+}
+
 // collideD per scene on the arrays gathered by the sort (the row-range walk
 // of the single system, collide_one_rows): the scene comes from the sorted
 // key in sortedVel.w, the cell tables are the scene's slice of the batch
-// tables, the parameters and row-walk constants the scene's own.
+// tables, the parameters and row-walk constants the scene's own.  The lanes
+// of a wave take their scenes one at a time (nearly every wave holds one
+// scene), each with its parameters in scalar registers, which keeps them
+// out of the vector registers the walk needs.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_collide_rows(
     const fc_sim_params* __restrict__ prm, const SceneCollide* __restrict__ sc, int cellBits,
@@ -168,17 +200,91 @@ __global__ __launch_bounds__(BS) void k_batch_collide_rows(
   if (k >= n) return;
   const float4 v4 = sortedVel[k];
   const uint32_t s = __float_as_uint(v4.w) >> cellBits;
-  const fc_sim_params P = prm[s];
-  const SceneCollide c = sc[s];
-  const size_t base = (size_t)s * C;
-  const float4 p4 = sortedPos[k];
-  const v3f pos = xyz(p4), vel = xyz(v4);
-  const v3f force = add_collider(
-      P, pos, vel, P.particleRadius,
-      collide_one_rows<false, BS>(P, c.invCell, c.contactD2, k, pos, vel, P.particleRadius,
-                                  sortedPos, sortedVel, sortedQ, c.qThreshold, cellStart + base,
-                                  cellEnd + base, n, s_list));
-  newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
+  bool pending = true;
+  for (uint64_t m = __ballot(pending); m != 0ull; m = __ballot(pending)) {
+    const uint32_t lead = lead_scene(m, s);
+    const fc_sim_params P = scene_block(prm, lead);  // (before the divergent branch)
+    const SceneCollide c = scene_block(sc, lead);
+    if (!pending || s != lead) continue;
+    pending = false;
+    const size_t base = (size_t)lead * C;
+    const float4 p4 = sortedPos[k];
+    const v3f pos = xyz(p4), vel = xyz(v4);
+    const v3f force = add_collider(
+        P, pos, vel, P.particleRadius,
+        collide_one_rows<false, BS>(P, c.invCell, c.contactD2, k, pos, vel, P.particleRadius,
+                                    sortedPos, sortedVel, sortedQ, c.qThreshold,
+                                    cellStart + base, cellEnd + base, n, s_list));
+    newVel[index[k]] = make_float4(vel.x + force.x, vel.y + force.y, vel.z + force.z, 0.0f);
+  }
+}
+
+// k_batch_collide_rows inside a multi-step run (fc_batch_run), where the
+// state between steps lives at the sorted slots: fused with the scene's next
+// integrate and hash, which the update order runs next on the same values.
+// The integrated state is written at the slot k, the next key at the
+// particle's id (keysOut[id], so the stable sort still orders ties by id),
+// slotOf[id] = k (the value the sort carries) and idOfSlot[k] = id, and the
+// keys' digit-0 histogram for the next sort.  kLast (the run's last step):
+// newVel and the positions go back to the ids instead.
+template <int BS, bool kLast>
+__global__ __launch_bounds__(BS) void k_batch_collide_run(
+    const fc_sim_params* __restrict__ prm, const SceneCollide* __restrict__ sc, int cellBits,
+    uint32_t C, float4* __restrict__ pos, float4* __restrict__ vel,
+    const float4* __restrict__ sortedPos, const float4* __restrict__ sortedVel,
+    const uint2* __restrict__ sortedQ, const uint32_t* __restrict__ index,
+    const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n,
+    float dt, uint32_t* __restrict__ keysOut, uint32_t* __restrict__ slotOf,
+    uint32_t* __restrict__ idOfSlot, int passes, uint32_t* gHist, uint32_t* counters) {
+  __shared__ uint32_t s_list[kListCap + 1][BS];
+  __shared__ uint32_t s_hist[kLast ? 1 : RS_RADIX];
+  const uint32_t k =
+      xcd_grouped_block(blockIdx.x, gridDim.x, kCollideXcdGroup * (256 / BS)) * BS + threadIdx.x;
+  const bool valid = k < n;
+  if (!kLast) {
+    for (int i = threadIdx.x; i < RS_RADIX; i += BS) s_hist[i] = 0u;
+    histogram_producer_prologue(gHist, passes, counters);
+  }
+  uint32_t key = 0;
+  const float4 v4 = valid ? sortedVel[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t s = __float_as_uint(v4.w) >> cellBits;
+  bool pending = valid;
+  for (uint64_t m = __ballot(pending); m != 0ull; m = __ballot(pending)) {
+    const uint32_t lead = lead_scene(m, s);
+    const fc_sim_params P = scene_block(prm, lead);  // (before the divergent branch)
+    const SceneCollide c = scene_block(sc, lead);
+    if (!pending || s != lead) continue;
+    pending = false;
+    const size_t base = (size_t)lead * C;
+    const float4 p4 = sortedPos[k];
+    const v3f p0 = xyz(p4), v0 = xyz(v4);
+    const v3f force = add_collider(
+        P, p0, v0, P.particleRadius,
+        collide_one_rows<false, BS>(P, c.invCell, c.contactD2, k, p0, v0, P.particleRadius,
+                                    sortedPos, sortedVel, sortedQ, c.qThreshold,
+                                    cellStart + base, cellEnd + base, n, s_list));
+    const uint32_t orig = index[k];
+    const v3f nv = make_v3(v0.x + force.x, v0.y + force.y, v0.z + force.z);
+    if (kLast) {
+      vel[orig] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+      pos[orig] = p4;
+    } else {
+      v3f p = p0, v = nv;
+      integrate_box(P, P.particleRadius, dt, p, v);
+      pos[k] = make_float4(p.x, p.y, p.z, p4.w);
+      vel[k] = make_float4(v.x, v.y, v.z, 0.0f);
+      key = (lead << cellBits) | hash_of(P, p.x, p.y, p.z);
+      keysOut[orig] = key;
+      slotOf[orig] = k;
+      idOfSlot[k] = orig;
+    }
+  }
+  if (!kLast) {
+    __syncthreads();
+    histogram_add_digit(s_hist, key & (RS_RADIX - 1), valid);
+    __syncthreads();
+    histogram_flush(s_hist, gHist, 0);
+  }
 }
 
 // Kinetic energy and momentum per scene in FP64, one block per scene, fixed
@@ -253,6 +359,28 @@ hipError_t launch_batch_collide_rows(fc_batch* b) {
   return hipGetLastError();
 }
 
+hipError_t launch_batch_collide_run(fc_batch* b, float dt, bool last, int passes) {
+  fc_context* ctx = b->ctx;
+  const int bs = b->tuning.collideBlock;
+#define FC_BCRUN(BS, LAST)                                                                     \
+  hipLaunchKernelGGL((k_batch_collide_run<BS, LAST>), dim3(blocks_for(b->n, BS)), dim3(BS), 0, \
+                     ctx->stream, b->dParams, b->dSceneCollide, b->cellBits, b->C, b->pos,     \
+                     b->vel, b->sortedPos, b->sortedVel, b->sortedQ, b->index, b->cellStart,   \
+                     b->cellEnd, b->n, dt, b->keys, b->slotOf, b->idOfSlot, passes,            \
+                     ctx->sort.histogram, ctx->sort.counters)
+  if (last) {
+    if (bs == 64) FC_BCRUN(64, true);
+    else if (bs == 128) FC_BCRUN(128, true);
+    else FC_BCRUN(256, true);
+  } else {
+    if (bs == 64) FC_BCRUN(64, false);
+    else if (bs == 128) FC_BCRUN(128, false);
+    else FC_BCRUN(256, false);
+  }
+#undef FC_BCRUN
+  return hipGetLastError();
+}
+
 // S * C < 2^32 is checked at creation.
 int batch_key_bits(const fc_batch* b) { return key_bits_for_cells(b->S * b->C); }
 
@@ -276,6 +404,10 @@ void drop_batch_graph(fc_batch* b) {
   if (b->graph) hipGraphDestroy(b->graph);
   b->graphExec = nullptr;
   b->graph = nullptr;
+  if (b->runGraphExec) hipGraphExecDestroy(b->runGraphExec);
+  if (b->runGraph) hipGraphDestroy(b->runGraph);
+  b->runGraphExec = nullptr;
+  b->runGraph = nullptr;
 }
 
 // One update of every scene: integrate+hash -> sort -> reorder -> collide.
@@ -342,6 +474,100 @@ fc_status capture_batch_graph(fc_batch* b, float dt) {
   return FC_SUCCESS;
 }
 
+// A multi-step run of the row-range path, as fc_system_run runs a system:
+// the head integrates and hashes from the ids; each step sorts (gathering
+// the state, from the ids after the head, from the slots after that),
+// writes the cell tables and collides, fused with the next integrate and
+// hash except in the last step, which returns the state to the ids.  Every
+// value is the one the update path computes.
+constexpr uint32_t kBatchRunSteps = 8;  // steady-state steps per graph launch
+
+fc_status enqueue_batch_run_step(fc_batch* b, float dt, bool fromSlots, bool last) {
+  fc_context* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const uint64_t cells = (uint64_t)b->S * b->C;
+  const int bits = batch_key_bits(b);
+  const int passes = radix_passes(bits, false);
+  SortGather ga;
+  ga.pos = b->pos;
+  ga.vel = b->vel;
+  ga.sortedPos = b->sortedPos;
+  ga.sortedVel = b->sortedVel;
+  ga.sortedQ = b->sortedQ;
+  ga.frame = batch_frame();
+  if (fromSlots) {
+    // the values are the slots; the ids come through idOfSlot; the first
+    // pass clears the previous step's cells
+    ga.idOfSlot = b->idOfSlot;
+    ga.indexOut = b->index;
+    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, b->slotOf, b->hash,
+                                b->slotOf, b->n, bits, nullptr, /*histogramReady=*/true, 2,
+                                b->hash, b->cellStart, b->cellEnd, &ga));
+  } else {
+    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash,
+                                b->index, b->n, bits, nullptr, /*histogramReady=*/true, 1,
+                                nullptr, nullptr, nullptr, &ga));
+  }
+  FC_HIP_TRY(launch_cell_bounds(st, b->cellStart, b->cellEnd, b->hash, b->n, (uint32_t)cells));
+  FC_HIP_TRY(launch_batch_collide_run(b, dt, last, passes));
+  return FC_SUCCESS;
+}
+
+fc_status capture_batch_run_graph(fc_batch* b, float dt) {
+  if (b->runGraphExec) hipGraphExecDestroy(b->runGraphExec);
+  if (b->runGraph) hipGraphDestroy(b->runGraph);
+  b->runGraphExec = nullptr;
+  b->runGraph = nullptr;
+  hipStream_t st = b->ctx->stream;
+  FC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  fc_status s = FC_SUCCESS;
+  for (uint32_t i = 0; i < kBatchRunSteps && s == FC_SUCCESS; ++i)
+    s = enqueue_batch_run_step(b, dt, /*fromSlots=*/true, /*last=*/false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != FC_SUCCESS) {
+    if (g) hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  b->runGraph = g;
+  e = hipGraphInstantiate(&b->runGraphExec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    b->runGraph = nullptr;
+    b->runGraphExec = nullptr;
+    return hip_error(e, "hipGraphInstantiate");
+  }
+  b->runGraphDt = dt;
+  return FC_SUCCESS;
+}
+
+fc_status enqueue_batch_run(fc_batch* b, float dt, uint32_t steps) {
+  fc_context* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const uint64_t cells = (uint64_t)b->S * b->C;
+  // head: integrate + hash from the ids, clearing the previous cells
+  const uint32_t* prev = nullptr;
+  if (b->tablesTrackHash) {
+    prev = b->hash;
+  } else {
+    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells));
+    FC_HIP_TRY(launch_fill_u32(st, b->cellEnd, CELL_EMPTY, cells));
+  }
+  FC_HIP_TRY(launch_batch_integrate_hash(b, dt, prev, radix_passes(batch_key_bits(b), false)));
+  b->tablesTrackHash = true;
+  FC_TRY(enqueue_batch_run_step(b, dt, /*fromSlots=*/false, /*last=*/steps == 1));
+  if (steps == 1) return FC_SUCCESS;
+  uint32_t middle = steps - 2;
+  if (middle >= kBatchRunSteps && (!b->runGraphExec || b->runGraphDt != dt))
+    FC_TRY(capture_batch_run_graph(b, dt));
+  for (; middle >= kBatchRunSteps; middle -= kBatchRunSteps)
+    FC_HIP_TRY(hipGraphLaunch(b->runGraphExec, st));
+  for (; middle > 0; --middle)
+    FC_TRY(enqueue_batch_run_step(b, dt, /*fromSlots=*/true, /*last=*/false));
+  return enqueue_batch_run_step(b, dt, /*fromSlots=*/true, /*last=*/true);
+}
+
 fc_status upload_batch_params(fc_batch* b) {
   std::vector<float> d2(b->S);
   std::vector<SceneCollide> sc(b->S);
@@ -376,6 +602,8 @@ void free_batch(fc_batch* b) {
   hipFree(b->hash);
   hipFree(b->index);
   hipFree(b->keys);
+  hipFree(b->slotOf);
+  hipFree(b->idOfSlot);
   hipFree(b->cellStart);
   hipFree(b->cellEnd);
   hipFree(b->stats);
@@ -437,6 +665,8 @@ fc_status fc_batch_create(fc_context* ctx, uint32_t numScenes, const uint32_t* s
   if (e == hipSuccess) e = hipMalloc(&b->hash, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->index, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->slotOf, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&b->idOfSlot, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->cellStart, cells * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->cellEnd, cells * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&b->stats, numScenes * 4 * sizeof(double));
@@ -544,6 +774,8 @@ fc_status fc_batch_update(fc_batch* b, float deltaTime) {
 
 fc_status fc_batch_run(fc_batch* b, float deltaTime, uint32_t steps) {
   if (!b) return set_error(FC_ERROR_INVALID_VALUE, "batch is NULL");
+  if (steps == 0 || b->n == 0) return FC_SUCCESS;
+  if (batch_rows_ok(b)) return enqueue_batch_run(b, deltaTime, steps);
   uint32_t i = 0;
   if (steps > 0 && !b->tablesTrackHash) {
     FC_TRY(enqueue_batch_update(b, deltaTime));
