@@ -170,3 +170,31 @@ def test_batch_full_size_ensemble_sampled(gpu_ctx):
         assert np.array_equal(bits(b.scene(gp, s)), bits(st.pos4)), s
         assert np.array_equal(bits(b.scene(gv, s)), bits(st.vel4)), s
     b.close()
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 9, 10, 17])
+def test_batch_run_lengths_equal_updates(gpu_ctx, steps):
+    """fc_batch_run keeps the state at the sorted slots between its steps,
+    fuses each collide with the next integrate, and launches eight steady
+    steps per graph; every run length (no steady step, one, a graph and a
+    remainder) must equal the same number of single updates, and a run
+    after updates and updates after a run must continue the same
+    trajectory."""
+    counts = np.array([700, 2500, 1, 4096, 0, 1900], np.uint32)
+    params = _scene_params(counts, seed=13)
+    print(f"[batch run] steps={steps}", flush=True)
+    a = _make(gpu_ctx, counts, params)
+    b = _make(gpu_ctx, counts, params)
+    for _ in range(steps):
+        a.update(0.5)
+    b.run(0.5, steps)
+    assert np.array_equal(bits(a.get_array(0)), bits(b.get_array(0)))
+    assert np.array_equal(bits(a.get_array(1)), bits(b.get_array(1)))
+    # continue: a run on the updated batch, updates on the run one
+    a.run(0.5, 3)
+    for _ in range(3):
+        b.update(0.5)
+    assert np.array_equal(bits(a.get_array(0)), bits(b.get_array(0)))
+    assert np.array_equal(bits(a.get_array(1)), bits(b.get_array(1)))
+    a.close()
+    b.close()
