@@ -44,6 +44,8 @@ namespace fc {
 namespace {
 
 constexpr int kWaves = RS_THREADS / 64;
+// Sorts of at least this many keys use 8192-key tiles (k_rs_scatter).
+constexpr uint32_t kWideTileKeys = 1u << 22;
 constexpr int kWaveKeys = 64 * RS_ITEMS;
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPrefix = 2ull << 62;
@@ -110,20 +112,25 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_histogram(
   histogram_flush(s_hist, gHist, 0);
 }
 
-// One LSD pass over one 4096-key tile per block (16 waves x 4 keys each).
+// One LSD pass over one tile of RS_THREADS x kItems keys per block (16
+// waves x kItems keys each): 4096-key tiles, or 8192-key tiles for large
+// sorts (radix_sort_pairs picks).
 // With ga.pos (the last pass of a sort that gathers the particle state) the
 // output loop also copies the state of each output's value to its final
 // position: the writes follow the keys' (digit-run) order.  Issuing these
 // loads with the keys instead and storing from the loading thread measured
 // 50 us per pass instead of 29 at 1M keys.
+template <int kItems>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int pass,
     int passes, uint32_t* gHist, uint64_t* status, uint64_t* groupStatus, uint32_t* counters,
     uint32_t* devFlags, const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
     uint32_t* __restrict__ cellEnd, SortGather ga, int ticketed) {
-  __shared__ uint32_t s_keys[RS_TILE];
-  __shared__ uint32_t s_vals[RS_TILE];
+  constexpr uint32_t kTile = (uint32_t)RS_THREADS * kItems;
+  constexpr uint32_t kTileWaveKeys = 64u * kItems;
+  __shared__ uint32_t s_keys[kTile];
+  __shared__ uint32_t s_vals[kTile];
   __shared__ uint32_t s_waveHist[kWaves][RS_RADIX];
   __shared__ uint32_t s_tileStart[RS_RADIX];
   __shared__ uint32_t s_digitBase[RS_RADIX];
@@ -131,7 +138,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   __shared__ uint32_t s_scratch[RS_RADIX / 64];
   __shared__ uint32_t s_tile;
 
-  const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
+  const uint32_t numTiles = (n + kTile - 1) / kTile;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
   const int shift = pass * RS_RADIX_BITS;
@@ -154,14 +161,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   __syncthreads();
   const uint32_t tile = ticketed ? s_tile : blockIdx.x;
   const uint32_t epoch = (epochBase * 4u + (uint32_t)pass) & kEpochMask;
-  const uint32_t tileBase = tile * RS_TILE;
-  const uint32_t waveBase = tileBase + wave * kWaveKeys;
+  const uint32_t tileBase = tile * kTile;
+  const uint32_t waveBase = tileBase + wave * kTileWaveKeys;
   if (clearHash != nullptr) {
     // Reset the previous update's cells: cell h was written iff it starts a
     // run of the previous sorted hashes.  clearHash is only rewritten by the
     // last pass, which runs after this one.
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
+    for (int i = 0; i < kItems; ++i) {
       const uint32_t idx = tileBase + i * RS_THREADS + threadIdx.x;
       if (idx < n) {
         const uint32_t h = clearHash[idx];
@@ -173,9 +180,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
   }
 
-  uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+  uint32_t key[kItems], val[kItems], rank[kItems];
 #pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
+  for (int i = 0; i < kItems; ++i) {
     const uint32_t idx = waveBase + i * 64u + lane;
     const bool valid = idx < n;
     key[i] = valid ? keysIn[idx] : 0xffffffffu;
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   // the next pass's digit.
   const bool countNext = pass + 1 < passes;
 #pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
+  for (int i = 0; i < kItems; ++i) {
     const uint32_t idx = waveBase + i * 64u + lane;
     const bool valid = idx < n;
     if (countNext)
@@ -305,12 +312,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
 
   // Stage the tile digit-sorted in LDS.
 #pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
+  for (int i = 0; i < kItems; ++i) {
     const uint32_t idx = waveBase + i * 64u + lane;
     if (idx < n) {
       const uint32_t dg = (key[i] >> shift) & (RS_RADIX - 1);
       const uint32_t lp = s_tileStart[dg] + s_waveHist[wave][dg] + rank[i];
-      if (lp < (uint32_t)RS_TILE) {
+      if (lp < (uint32_t)kTile) {
         s_keys[lp] = key[i];
         s_vals[lp] = val[i];
       }
@@ -318,15 +325,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   }
   __syncthreads();
 
-  const uint32_t validCount = (n - tileBase) < (uint32_t)RS_TILE ? (n - tileBase) : (uint32_t)RS_TILE;
+  const uint32_t validCount = (n - tileBase) < (uint32_t)kTile ? (n - tileBase) : (uint32_t)kTile;
   // Output: every item's key, value and destination first, then (the last
   // pass of a gathering sort) all of the items' state loads together, then
   // the stores.  The state arrays may alias nothing here, but the compiler
   // cannot know it: loads placed after a store wait for the store's round
   // trip, which serialised the gather.
-  uint32_t ok[RS_ITEMS], ks[RS_ITEMS], vs[RS_ITEMS], gs[RS_ITEMS];
+  uint32_t ok[kItems], ks[kItems], vs[kItems], gs[kItems];
 #pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
+  for (int i = 0; i < kItems; ++i) {
     const uint32_t s = i * RS_THREADS + threadIdx.x;
     ok[i] = 0u;
     ks[i] = vs[i] = gs[i] = 0u;
@@ -339,10 +346,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
   }
   if (ga.pos) {
-    float4 pv[RS_ITEMS], vv[RS_ITEMS];
-    uint32_t id[RS_ITEMS];
+    float4 pv[kItems], vv[kItems];
+    uint32_t id[kItems];
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
+    for (int i = 0; i < kItems; ++i) {
       const bool live = ok[i] && vs[i] < n;  // v < n: values index the particle arrays
       const uint32_t v = live ? vs[i] : 0u;
       pv[i] = ga.pos[v];
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       id[i] = ga.idOfSlot ? ga.idOfSlot[v] : 0u;
     }
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
+    for (int i = 0; i < kItems; ++i) {
       if (!ok[i]) continue;
       const uint32_t g = gs[i];
       keysOut[g] = ks[i];
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
+    for (int i = 0; i < kItems; ++i) {
       if (!ok[i]) continue;
       keysOut[gs[i]] = ks[i];
       valsOut[gs[i]] = vs[i];
@@ -573,7 +580,13 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
   if (passes < minPasses) passes = minPasses;
   if (clearHash != nullptr && passes < 2) return hipErrorInvalidValue;
   if (passes > RS_MAX_PASSES) return hipErrorInvalidValue;
-  const uint32_t numTiles = (n + RS_TILE - 1) / RS_TILE;
+  // Large sorts take 8192-key tiles: half the tiles and look-back chain,
+  // longer digit runs per tile.  The batched ensemble's step (four passes
+  // over 11.9M keys) measured 1.656 against 1.735 ms with them; at 1M keys
+  // a pass took 21.9 against 18.5 us.
+  const bool wide = n >= kWideTileKeys;
+  const uint32_t tileKeys = wide ? 2u * (uint32_t)RS_TILE : (uint32_t)RS_TILE;
+  const uint32_t numTiles = (n + tileKeys - 1) / tileKeys;
   if (s.capacity < n || s.statusCapacity < (size_t)numTiles * RS_RADIX ||
       s.groupStatusCapacity < (size_t)lookback_groups(numTiles) * RS_RADIX)
     return hipErrorInvalidValue;
@@ -609,11 +622,18 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
     static const char* names[RS_MAX_PASSES] = {"sort_pass0", "sort_pass1", "sort_pass2",
                                                "sort_pass3"};
     if (timer) timer->begin(stream, names[p]);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(numTiles), dim3(RS_THREADS), 0, stream, kin, vin, kout,
-                       vout, n, p, passes, s.histogram, s.status, s.groupStatus, s.counters,
-                       devFlags, p == 0 ? clearHash : nullptr, cellStart, cellEnd,
-                       (gather && p == passes - 1) ? *gather : SortGather{},
-                       numTiles > kUnticketedTiles ? 1 : 0);
+    const SortGather ga = (gather && p == passes - 1) ? *gather : SortGather{};
+    const int ticketed = numTiles > kUnticketedTiles ? 1 : 0;
+    if (wide)
+      hipLaunchKernelGGL(k_rs_scatter<2 * RS_ITEMS>, dim3(numTiles), dim3(RS_THREADS), 0, stream,
+                         kin, vin, kout, vout, n, p, passes, s.histogram, s.status,
+                         s.groupStatus, s.counters, devFlags, p == 0 ? clearHash : nullptr,
+                         cellStart, cellEnd, ga, ticketed);
+    else
+      hipLaunchKernelGGL(k_rs_scatter<RS_ITEMS>, dim3(numTiles), dim3(RS_THREADS), 0, stream,
+                         kin, vin, kout, vout, n, p, passes, s.histogram, s.status,
+                         s.groupStatus, s.counters, devFlags, p == 0 ? clearHash : nullptr,
+                         cellStart, cellEnd, ga, ticketed);
     e = hipGetLastError();
     if (timer) timer->end(stream);
     if (e != hipSuccess) return e;

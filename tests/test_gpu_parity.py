@@ -213,7 +213,8 @@ def test_eval_forces_bitexact(gpu_ctx, case):
     assert not np.any(differ[dist < reach * (1 - 1e-6)] == False)  # noqa: E712
 
 
-@pytest.mark.parametrize("n", [1, 2, 255, 4095, 4096, 4097, 100_003, 1 << 20])
+# (from 2^22 keys the passes take 8192-key tiles)
+@pytest.mark.parametrize("n", [1, 2, 255, 4095, 4096, 4097, 100_003, 1 << 20, (1 << 22) + 4099])
 def test_sort_24bit_keys_bitexact(gpu_ctx, n):
     print(f"[sort] n={n}", flush=True)
     P = sim_params(n, 256, 1.0 / 128.0)
