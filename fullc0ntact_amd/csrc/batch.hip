@@ -14,6 +14,7 @@
 #include "fc_device.hpp"
 #include "radix_common.hpp"
 #include "collide.hpp"
+#include "row_resort.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -60,6 +61,9 @@ struct fc_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   float graphDt = 0.0f;
+  // Row resort of a multi-step run's steady steps (rows: S x gridSize.y x
+  // gridSize.z, each gridSize.x consecutive keys)
+  fc::RowTables rowTables;
   // kBatchRunSteps steady-state steps of a multi-step run
   hipGraph_t runGraph = nullptr;
   hipGraphExec_t runGraphExec = nullptr;
@@ -235,13 +239,17 @@ __global__ __launch_bounds__(BS) void k_batch_collide_run(
     const uint2* __restrict__ sortedQ, const uint32_t* __restrict__ index,
     const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellEnd, uint32_t n,
     float dt, uint32_t* __restrict__ keysOut, uint32_t* __restrict__ slotOf,
-    uint32_t* __restrict__ idOfSlot, int passes, uint32_t* gHist, uint32_t* counters) {
+    uint32_t* __restrict__ idOfSlot, int passes, uint32_t* gHist, uint32_t* counters,
+    RowHists rows) {
   __shared__ uint32_t s_list[kListCap + 1][BS];
   __shared__ uint32_t s_hist[kLast ? 1 : RS_RADIX];
   const uint32_t k =
       xcd_grouped_block(blockIdx.x, gridDim.x, kCollideXcdGroup * (256 / BS)) * BS + threadIdx.x;
   const bool valid = k < n;
-  if (!kLast) {
+  // rows.rowHist: the next sort is a row resort (next keys at the slots,
+  // per-row counts) instead of a radix sort (keys at the ids, digit counts)
+  const bool slotKeys = rows.rowHist != nullptr;
+  if (!kLast && !slotKeys) {
     for (int i = threadIdx.x; i < RS_RADIX; i += BS) s_hist[i] = 0u;
     histogram_producer_prologue(gHist, passes, counters);
   }
@@ -274,16 +282,26 @@ __global__ __launch_bounds__(BS) void k_batch_collide_run(
       pos[k] = make_float4(p.x, p.y, p.z, p4.w);
       vel[k] = make_float4(v.x, v.y, v.z, 0.0f);
       key = (lead << cellBits) | hash_of(P, p.x, p.y, p.z);
-      keysOut[orig] = key;
-      slotOf[orig] = k;
+      if (slotKeys) {
+        // at the slot, with whether the sphere changed cell (v4.w holds its
+        // current key)
+        keysOut[k] = key != __float_as_uint(v4.w) ? (key | kMovedBit) : key;
+      } else {
+        keysOut[orig] = key;
+        slotOf[orig] = k;
+      }
       idOfSlot[k] = orig;
     }
   }
   if (!kLast) {
-    __syncthreads();
-    histogram_add_digit(s_hist, key & (RS_RADIX - 1), valid);
-    __syncthreads();
-    histogram_flush(s_hist, gHist, 0);
+    if (slotKeys) {
+      row_histograms(rows, k, key, __float_as_uint(v4.w), valid);
+    } else {
+      __syncthreads();
+      histogram_add_digit(s_hist, key & (RS_RADIX - 1), valid);
+      __syncthreads();
+      histogram_flush(s_hist, gHist, 0);
+    }
   }
 }
 
@@ -359,7 +377,8 @@ hipError_t launch_batch_collide_rows(fc_batch* b) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch_collide_run(fc_batch* b, float dt, bool last, int passes) {
+hipError_t launch_batch_collide_run(fc_batch* b, float dt, bool last, int passes,
+                                   const RowHists& rows) {
   fc_context* ctx = b->ctx;
   const int bs = b->tuning.collideBlock;
 #define FC_BCRUN(BS, LAST)                                                                     \
@@ -367,7 +386,7 @@ hipError_t launch_batch_collide_run(fc_batch* b, float dt, bool last, int passes
                      ctx->stream, b->dParams, b->dSceneCollide, b->cellBits, b->C, b->pos,     \
                      b->vel, b->sortedPos, b->sortedVel, b->sortedQ, b->index, b->cellStart,   \
                      b->cellEnd, b->n, dt, b->keys, b->slotOf, b->idOfSlot, passes,            \
-                     ctx->sort.histogram, ctx->sort.counters)
+                     ctx->sort.histogram, ctx->sort.counters, rows)
   if (last) {
     if (bs == 64) FC_BCRUN(64, true);
     else if (bs == 128) FC_BCRUN(128, true);
@@ -474,42 +493,96 @@ fc_status capture_batch_graph(fc_batch* b, float dt) {
   return FC_SUCCESS;
 }
 
+// The batch's rows as the row resort sees them: kBatchRowGridRows
+// consecutive grid rows of a scene form one row (a contiguous range of
+// keys), so row = key >> log2(gridSize.x * kBatchRowGridRows).  The
+// resort's cost is a chain of dependent loads per row, and a scene's grid
+// rows hold few spheres (about 20 on the freshly dropped 64^3 ensemble):
+// per ensemble step, 1 grid row per row measured 1.876 ms, 4 1.391 ms,
+// 8 1.323 ms (radix passes: 1.663 ms); 16 and 32 (4.6 and 9.1 ms) put
+// rows past the row sort's LDS capacity (768 members) into its counting
+// path.  A settled bed (about 2.2 spheres per occupied cell, some 140 per
+// 64-cell grid row) would do that to 8 as well; 4 keeps such rows near
+// 570 members.
+constexpr uint32_t kBatchRowGridRows = 4;
+
+fc_sim_params batch_row_grid(const fc_batch* b) {
+  const uint32_t f = b->grid.y < kBatchRowGridRows ? b->grid.y : kBatchRowGridRows;
+  fc_sim_params P = b->params[0];
+  P.gridSize = fc_uint3{b->grid.x * f, b->grid.y / f, b->grid.z * b->S};
+  return P;
+}
+
+// Whether a run's steady steps are row resorts (the row tables fit).
+bool batch_row_mode(const fc_batch* b) {
+  return batch_rows_ok(b) && (uint64_t)b->grid.z * b->S <= 0xffffffffull &&
+         row_resort_grid_ok(batch_row_grid(b));
+}
+
 // A multi-step run of the row-range path, as fc_system_run runs a system:
 // the head integrates and hashes from the ids; each step sorts (gathering
-// the state, from the ids after the head, from the slots after that),
-// writes the cell tables and collides, fused with the next integrate and
-// hash except in the last step, which returns the state to the ids.  Every
-// value is the one the update path computes.
+// the state), writes the cell tables and collides, fused with the next
+// integrate and hash except in the last step, which returns the state to
+// the ids.  The first step's sort is a radix sort from the ids; in row mode
+// each later one is a row resort of the next keys the fused collide stored
+// at the slots (row_resort.hip), else a radix sort gathering from the
+// previous step's slots.  Every value is the one the update path computes.
 constexpr uint32_t kBatchRunSteps = 8;  // steady-state steps per graph launch
 
+// keysAtSlots: the keys of this step's sort were stored at the slots (a
+// steady step in row mode).  fromSlots: the state is at the previous step's
+// slots (every step but the first).
 fc_status enqueue_batch_run_step(fc_batch* b, float dt, bool fromSlots, bool last) {
   fc_context* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const uint64_t cells = (uint64_t)b->S * b->C;
   const int bits = batch_key_bits(b);
   const int passes = radix_passes(bits, false);
-  SortGather ga;
-  ga.pos = b->pos;
-  ga.vel = b->vel;
-  ga.sortedPos = b->sortedPos;
-  ga.sortedVel = b->sortedVel;
-  ga.sortedQ = b->sortedQ;
-  ga.frame = batch_frame();
-  if (fromSlots) {
-    // the values are the slots; the ids come through idOfSlot; the first
-    // pass clears the previous step's cells
-    ga.idOfSlot = b->idOfSlot;
-    ga.indexOut = b->index;
-    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, b->slotOf, b->hash,
-                                b->slotOf, b->n, bits, nullptr, /*histogramReady=*/true, 2,
-                                b->hash, b->cellStart, b->cellEnd, &ga));
+  const bool rowMode = batch_row_mode(b);
+  RowTables& T = b->rowTables;
+  if (rowMode && fromSlots) {
+    FC_HIP_TRY(launch_row_scan(st, T, ctx->devFlags));
+    FC_HIP_TRY(launch_row_changers(st, T, b->keys, b->hash, b->cellStart, b->cellEnd, b->n,
+                                   (uint32_t)cells, ctx->devFlags));
+    RowOutputs O;
+    O.hash = b->hash;
+    O.index = b->index;
+    O.pos = b->pos;
+    O.vel = b->vel;
+    O.sortedPos = b->sortedPos;
+    O.sortedVel = b->sortedVel;
+    O.sortedQ = b->sortedQ;
+    O.cellStart = b->cellStart;
+    O.cellEnd = b->cellEnd;
+    O.frame = batch_frame();
+    FC_HIP_TRY(launch_row_sort(st, T, b->keys, b->idOfSlot, O, ctx->devFlags));
   } else {
-    FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash,
-                                b->index, b->n, bits, nullptr, /*histogramReady=*/true, 1,
-                                nullptr, nullptr, nullptr, &ga));
+    SortGather ga;
+    ga.pos = b->pos;
+    ga.vel = b->vel;
+    ga.sortedPos = b->sortedPos;
+    ga.sortedVel = b->sortedVel;
+    ga.sortedQ = b->sortedQ;
+    ga.frame = batch_frame();
+    if (fromSlots) {
+      // the values are the slots; the ids come through idOfSlot; the first
+      // pass clears the previous step's cells
+      ga.idOfSlot = b->idOfSlot;
+      ga.indexOut = b->index;
+      FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, b->slotOf, b->hash,
+                                  b->slotOf, b->n, bits, nullptr, /*histogramReady=*/true, 2,
+                                  b->hash, b->cellStart, b->cellEnd, &ga));
+    } else {
+      FC_HIP_TRY(radix_sort_pairs(st, ctx->sort, ctx->devFlags, b->keys, nullptr, b->hash,
+                                  b->index, b->n, bits, nullptr, /*histogramReady=*/true, 1,
+                                  nullptr, nullptr, nullptr, &ga));
+    }
+    FC_HIP_TRY(launch_cell_bounds(st, b->cellStart, b->cellEnd, b->hash, b->n, (uint32_t)cells));
+    // the next step's row resort starts from this order's rows
+    if (rowMode) FC_HIP_TRY(launch_row_starts(st, T, b->hash, b->n));
   }
-  FC_HIP_TRY(launch_cell_bounds(st, b->cellStart, b->cellEnd, b->hash, b->n, (uint32_t)cells));
-  FC_HIP_TRY(launch_batch_collide_run(b, dt, last, passes));
+  FC_HIP_TRY(launch_batch_collide_run(b, dt, last, passes,
+                                      rowMode && !last ? row_hists(T) : RowHists{}));
   return FC_SUCCESS;
 }
 
@@ -546,6 +619,8 @@ fc_status enqueue_batch_run(fc_batch* b, float dt, uint32_t steps) {
   fc_context* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const uint64_t cells = (uint64_t)b->S * b->C;
+  const bool rowMode = batch_row_mode(b);
+  if (rowMode) FC_TRY(ensure_row_tables(b->rowTables, batch_row_grid(b), b->n));
   // head: integrate + hash from the ids, clearing the previous cells
   const uint32_t* prev = nullptr;
   if (b->tablesTrackHash) {
@@ -554,6 +629,9 @@ fc_status enqueue_batch_run(fc_batch* b, float dt, uint32_t steps) {
     FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells));
     FC_HIP_TRY(launch_fill_u32(st, b->cellEnd, CELL_EMPTY, cells));
   }
+  // the row counts start from zero (a run that stopped early may have left
+  // counts behind)
+  if (rowMode) FC_HIP_TRY(launch_row_clear(st, b->rowTables));
   FC_HIP_TRY(launch_batch_integrate_hash(b, dt, prev, radix_passes(batch_key_bits(b), false)));
   b->tablesTrackHash = true;
   FC_TRY(enqueue_batch_run_step(b, dt, /*fromSlots=*/false, /*last=*/steps == 1));
@@ -607,6 +685,7 @@ void free_batch(fc_batch* b) {
   hipFree(b->cellStart);
   hipFree(b->cellEnd);
   hipFree(b->stats);
+  free_row_tables(b->rowTables);
 }
 
 bool valid_tuning(const fc_batch_tuning& t) {
