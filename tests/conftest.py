@@ -1,7 +1,7 @@
 """Shared test setup.
 
 Markers:
-  gpu  - needs a HIP device (MI355X); run with `pytest -m gpu` on the GPU box.
+  gpu  - needs a HIP device (MI355X); run with `pytest -m gpu` on a machine with one.
 Everything else runs on the CPU: the oracle against closed forms, invariants
 and golden fixtures, host logic, and the C ABI library's exports.
 """
@@ -37,7 +37,7 @@ _ensure_built()
 def gpu_ctx():
     from fullc0ntact_amd import _lib
     if _lib.device_count() == 0:
-        pytest.fail("no HIP device visible: GPU tests must run on the GPU box")
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
     ctx = _lib.Context(0)
     yield ctx
     ctx.close()

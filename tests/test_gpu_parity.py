@@ -120,7 +120,7 @@ class Stages:
 
 
 CASES = {
-    # 4096 spheres r=1/64, grid 64^3 (collide: the row-split kernel)
+    # the 4096-sphere case: r=1/64, grid 64^3 (collide: the row-split kernel)
     "cfg0": dict(n=4096, grid=64, radius=1.0 / 64.0),
     # ragged size, not a multiple of any block or tile
     "ragged": dict(n=5003, grid=64, radius=1.0 / 64.0),
