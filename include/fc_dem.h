@@ -433,8 +433,11 @@ fc_status fc_batch_reset(fc_batch* batch, fc_particle_config config, uint64_t se
                          float jitterFrac, float velAmp);
 /* One update of every scene. */
 fc_status fc_batch_update(fc_batch* batch, float deltaTime);
-/* steps updates, the steady step graph-launched; identical to repeated
- * fc_batch_update. */
+/* steps updates, identical to repeated fc_batch_update: between its steps
+ * the state stays in sorted order, each collide fused with the next
+ * integrate, the later steps' sorts row resorts, eight steps per graph
+ * launch; the state is back in particle order on return.  The first call
+ * allocates the row tables (a call that cannot be captured in a graph). */
 fc_status fc_batch_run(fc_batch* batch, float deltaTime, uint32_t steps);
 /* Per scene {kinetic energy, px, py, pz} in binary64 (host double[4 *
  * numScenes]). */
