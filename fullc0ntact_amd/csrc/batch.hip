@@ -542,8 +542,8 @@ fc_status enqueue_batch_run_step(fc_batch* b, float dt, bool fromSlots, bool las
   RowTables& T = b->rowTables;
   if (rowMode && fromSlots) {
     FC_HIP_TRY(launch_row_scan(st, T, ctx->devFlags));
-    FC_HIP_TRY(launch_row_changers(st, T, b->keys, b->hash, b->cellStart, b->cellEnd, b->n,
-                                   (uint32_t)cells, ctx->devFlags));
+    FC_HIP_TRY(launch_row_changers(st, T, b->keys, b->idOfSlot, b->hash, b->cellStart,
+                                   b->cellEnd, b->n, (uint32_t)cells, ctx->devFlags));
     RowOutputs O;
     O.hash = b->hash;
     O.index = b->index;

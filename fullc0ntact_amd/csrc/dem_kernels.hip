@@ -42,7 +42,9 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
   if (tid < count - tail) p[tail + tid] = value;
 }
 
-// Plain 16-byte-per-lane copy (bandwidth probe).
+// Plain 16-byte-per-lane copy (bandwidth probe).  Launched with one lane per
+// element: on MI355X a capped grid that strides over 1 GiB (2048 or 8192
+// blocks) copies at 4.5-4.8 TB/s, one element per lane at 6.2 TB/s.
 __global__ __launch_bounds__(kBlock) void k_copy16(uint4* __restrict__ dst,
                                                    const uint4* __restrict__ src, size_t count) {
   size_t stride = (size_t)gridDim.x * kBlock;
@@ -481,8 +483,8 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t co
 hipError_t launch_copy16(hipStream_t s, void* dst, const void* src, size_t bytes) {
   const size_t count = bytes / 16;
   if (count == 0) return hipSuccess;
-  unsigned grid = grid_for(count, kBlock);
-  if (grid > 8192u) grid = 8192u;
+  const size_t blocks = (count + kBlock - 1) / kBlock;
+  const unsigned grid = blocks < (1u << 30) ? (unsigned)blocks : (1u << 30);
   hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, (uint4*)dst, (const uint4*)src,
                      count);
   return hipGetLastError();

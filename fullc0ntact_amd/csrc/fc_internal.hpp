@@ -330,10 +330,13 @@ namespace fc {
 struct RowTables {
   uint32_t rows = 0, shift = 0, groups = 0, capacity = 0;
   uint32_t* block = nullptr;
-  uint32_t* startCur = nullptr;   // [rows + 1] row starts of the current order
+  uint32_t* count = nullptr;      // [rows] spheres per row in the current order
   uint32_t* startNext = nullptr;  // [rows + 1] row starts of the next order
   uint32_t* chgStart = nullptr;   // [rows x copies + 1] bucket starts of the row changers
-  uint32_t* rowList = nullptr;    // [rows] the rows that hold spheres in the next order
+  // [rows][2] one entry per row that holds spheres in the next order, in row
+  // order: {row, next start, next count, current start}, {current count,
+  // changers' first bucket start, changers moving in, 0}
+  uint4* rowList = nullptr;
   uint64_t* groupWords = nullptr; // [groups][3] per 256-row group: spheres, changers, rows (epoch-tagged)
   uint32_t* listCount = nullptr;  // [1] entries of rowList
   uint32_t* epoch = nullptr;      // [1] steps resorted (tags the group words)
@@ -342,7 +345,8 @@ struct RowTables {
   uint32_t* rowHist = nullptr;    // [rows x copies] next keys per row
   uint32_t* chgHist = nullptr;    // [rows x copies] of those, the spheres whose row changes
   uint32_t* chgCursor = nullptr;  // [rows x copies]
-  uint32_t* chgList = nullptr;    // [capacity] current slots of the row changers, by new row
+  // [capacity] the row changers, by new row: {current slot, next key, id, 0}
+  uint4* chgList = nullptr;
 };
 
 // Where the row sort writes the sorted order: hash and id per position,
@@ -372,8 +376,9 @@ hipError_t launch_row_scan(hipStream_t s, const RowTables& T, uint32_t* devFlags
 hipError_t launch_row_starts(hipStream_t s, const RowTables& T, const uint32_t* sortedHash,
                              uint32_t n);
 hipError_t launch_row_changers(hipStream_t s, const RowTables& T, const uint32_t* keys,
-                               const uint32_t* hash, uint32_t* cellStart, uint32_t* cellEnd,
-                               uint32_t n, uint32_t numCells, uint32_t* devFlags);
+                               const uint32_t* idOfSlot, const uint32_t* hash,
+                               uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
+                               uint32_t numCells, uint32_t* devFlags);
 hipError_t launch_row_sort(hipStream_t s, const RowTables& T, const uint32_t* keys,
                            const uint32_t* idOfSlot, const RowOutputs& O, uint32_t* devFlags);
 }  // namespace fc

@@ -250,8 +250,8 @@ fc_status enqueue_run_body(fc_system* sys, float dt, bool clearInSort, bool last
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "row scan");
     if (timer) timer->begin(st, "row_changers");
-    e = launch_row_changers(st, sys->rowTables, sys->keys, sys->hash, sys->cellStart,
-                            sys->cellEnd, n, P.numCells, ctx->devFlags);
+    e = launch_row_changers(st, sys->rowTables, sys->keys, sys->idOfSlot, sys->hash,
+                            sys->cellStart, sys->cellEnd, n, P.numCells, ctx->devFlags);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "row changers");
     RowOutputs O;

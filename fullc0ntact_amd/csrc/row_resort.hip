@@ -35,9 +35,10 @@ namespace fc {
 namespace {
 
 constexpr int kScanThreads = 256;
-// The row sort: one row per 128-thread block, 9 KB of LDS, so that 16
-// blocks fit a CU and the ~3800 occupied rows of a 1M-sphere bed are all in
-// flight at once (each row is a chain of dependent loads).
+// The row sort: one row per 128-thread block, 9 KB of LDS and at most 64
+// VGPRs, so that 16 blocks fit a CU and the ~3800 occupied rows of a
+// 1M-sphere bed are all in flight at once (each row is a chain of dependent
+// loads).
 constexpr int kSortThreads = 128;
 constexpr uint32_t kRowCap = 768;      // members of a row held in LDS
 constexpr uint32_t kMoverCap = 256;    // movers ranked by counting
@@ -106,10 +107,10 @@ constexpr uint32_t kScanSpinLimit = 1u << 22;
 // appear (every group publishes before it waits, and waits only on earlier
 // ones; the wait is bounded and reported).  Each row then takes its start
 // in the next order (base + prefix within the group) and each (row, copy)
-// its changers' bucket start; the row's current start (startNext, from the
-// previous step) moves to startCur before it is overwritten; occupied
-// rows are listed in row order; the counters are zeroed for the next
-// producer.
+// its changers' bucket start; every occupied row gets its entry in row
+// order (next start and count, current start and count, the changers'
+// first bucket start and number), so that the row sort reads one entry
+// per row; the counters are zeroed for the next producer.
 __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T, uint32_t* devFlags) {
   __shared__ uint32_t s_wave[3][kScanThreads / 64];
   const uint32_t g = blockIdx.x;
@@ -117,8 +118,11 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T, uint32_t
   const uint32_t r = (g << kRowGroupShift) + threadIdx.x;
   const bool in = r < T.rows;
   uint32_t x[kRowCopies], y[kRowCopies];
-  uint32_t h = 0, c = 0;
+  uint32_t h = 0, c = 0, curStart = 0, curCount = 0;
   if (in) {
+    // the row's current start and count: only this thread rewrites them
+    curStart = T.startNext[r];
+    curCount = T.count[r];
     load_copies(T.rowHist, r, x);
     load_copies(T.chgHist, r, y);
 #pragma unroll
@@ -170,9 +174,12 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T, uint32_t
   pc += bc;
   po += bo;
   if (in) {
-    T.startCur[r] = T.startNext[r];
     T.startNext[r] = ph;
-    if (h != 0u) T.rowList[po] = r;
+    T.count[r] = h;
+    if (h != 0u) {
+      T.rowList[2 * (size_t)po] = make_uint4(r, ph, h, curStart);
+      T.rowList[2 * (size_t)po + 1] = make_uint4(curCount, pc, c, 0u);
+    }
     uint32_t st[kRowCopies];
 #pragma unroll
     for (uint32_t q = 0; q < kRowCopies; ++q) {
@@ -188,7 +195,6 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(RowTables T, uint32_t
   }
   if (g == T.groups - 1u && threadIdx.x == kScanThreads - 1) {
     // the last thread's prefixes plus its own counts are the totals
-    T.startCur[T.rows] = T.startNext[T.rows];
     T.startNext[T.rows] = ph + h;  // past the last row: the sphere count
     T.chgStart[(size_t)T.rows * kRowCopies] = pc;
     *T.listCount = po + (h != 0u ? 1u : 0u);
@@ -204,36 +210,42 @@ __global__ __launch_bounds__(256) void k_row_reset(RowTables T) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *T.epoch += 2u;
 }
 
-// Row starts of a sorted hash array: the first position whose hash lies in
-// row r or beyond (a binary search per row).
+// First position of a sorted hash array whose hash lies in row r or beyond.
+__device__ __forceinline__ uint32_t row_start_in(const uint32_t* hash, uint32_t n, uint32_t rows,
+                                                 uint32_t shift, uint32_t r) {
+  if (r >= rows) return n;
+  const uint32_t first = r << shift;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (hash[mid] < first)
+      lo = mid + 1u;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Row starts and counts of a sorted hash array (binary searches per row).
 __global__ __launch_bounds__(256) void k_row_starts(RowTables T, const uint32_t* __restrict__ hash,
                                                     uint32_t n) {
   const uint32_t r = blockIdx.x * 256u + threadIdx.x;
   if (r > T.rows) return;
-  uint32_t lo = 0, hi = n;
-  if (r == T.rows) {
-    lo = n;
-  } else {
-    const uint32_t first = r << T.shift;
-    while (lo < hi) {
-      const uint32_t mid = lo + ((hi - lo) >> 1);
-      if (hash[mid] < first)
-        lo = mid + 1u;
-      else
-        hi = mid;
-    }
-  }
+  const uint32_t lo = row_start_in(hash, n, T.rows, T.shift, r);
   T.startNext[r] = lo;
+  if (r < T.rows) T.count[r] = row_start_in(hash, n, T.rows, T.shift, r + 1u) - lo;
 }
 
 // One thread per current sorted entry k (keys[k] is the next key with
 // kMovedBit when the cell changes, hash[k] the current one).  A sphere
-// whose row changes is listed in its new row's bucket.  A current cell
+// whose row changes is listed in its new row's bucket with its slot, next
+// key and id.  A current cell
 // none of whose spheres stays is cleared in the tables (the row sort, which
 // runs next, writes every cell of the next order, this one too if spheres
 // move in): the wave's ballots tell a cell's first lane whether any of the
 // cell's lanes stays; only a cell that goes on past the wave reads on.
 __global__ __launch_bounds__(256) void k_row_changers(RowTables T, const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ idOfSlot,
                                                       const uint32_t* __restrict__ hash,
                                                       uint32_t* __restrict__ cellStart,
                                                       uint32_t* __restrict__ cellEnd, uint32_t n,
@@ -271,10 +283,11 @@ __global__ __launch_bounds__(256) void k_row_changers(RowTables T, const uint32_
     atomicOr(devFlags, DEVERR_ROW_COUNT);
     return;
   }
+  const uint32_t id = idOfSlot[k];  // loaded while the bucket position is taken
   const uint32_t b = r * kRowCopies + row_copy(k);
   const uint32_t at = T.chgStart[b] + atomicAdd(&T.chgCursor[b], 1u);
   if (at < T.chgStart[b + 1])
-    T.chgList[at] = k;
+    T.chgList[at] = make_uint4(k, raw & ~kMovedBit, id, 0u);
   else
     atomicOr(devFlags, DEVERR_ROW_COUNT);
 }
@@ -343,11 +356,17 @@ __device__ __forceinline__ uint64_t candidate(const RowTables& T, const uint32_t
                                               const uint32_t* idOfSlot, uint32_t r, uint32_t a0,
                                               uint32_t A, uint32_t c0, uint32_t u,
                                               uint32_t* slotOut) {
-  const uint32_t k = u < A ? a0 + u : T.chgList[c0 + (u - A)];
-  const uint32_t key = keys[k] & ~kMovedBit;
-  *slotOut = k;
-  if ((key >> T.shift) != r) return kNoMember;
-  return member_key(key, T.shift, idOfSlot[k]);
+  if (u < A) {
+    const uint32_t k = a0 + u;
+    const uint32_t key = keys[k] & ~kMovedBit;
+    *slotOut = k;
+    if ((key >> T.shift) != r) return kNoMember;
+    return member_key(key, T.shift, idOfSlot[k]);
+  }
+  const uint4 c = T.chgList[c0 + (u - A)];
+  *slotOut = c.x;
+  if ((c.y >> T.shift) != r) return kNoMember;
+  return member_key(c.y, T.shift, c.z);
 }
 
 // Number of entries of the ascending s_key[lo, lo + len) below x.
@@ -416,67 +435,99 @@ __device__ void rank_by_counting(const RowTables& T, const uint32_t* keys,
 
 // The next keys carry kMovedBit; the sorted hashes are written to O.hash
 // while other blocks still read their rows, so the current hashes are not
-// read here.
-__global__ __launch_bounds__(kSortThreads) void k_row_sort(RowTables T,
-                                                           const uint32_t* __restrict__ keys,
-                                                           const uint32_t* __restrict__ idOfSlot,
-                                                           RowOutputs O, uint32_t* devFlags) {
+// read here.  A row's loads form a short chain: its entry (read before the
+// list length, as the launch has at most one block per row), then its
+// current range's keys and ids together with its incomers' records, then,
+// after the merge in LDS, the members' state.
+__global__ __launch_bounds__(kSortThreads) void k_row_sort(
+    RowTables T, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ idOfSlot,
+    RowOutputs O, uint32_t* devFlags) {
   // pool: the stayers in order from the front, the movers from the back;
   // after the merge the row's members in order from the front
   __shared__ uint64_t s_key[kRowCap];
   __shared__ uint32_t s_slot[kRowCap];
   __shared__ uint32_t s_wave[kSortThreads / 64];
+  __shared__ uint32_t s_count[kPerThread][kSortThreads / 64];  // stayers per chunk and wave
   __shared__ uint32_t s_movers;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  uint4 e0 = T.rowList[2 * (size_t)blockIdx.x], e1 = T.rowList[2 * (size_t)blockIdx.x + 1];
   const uint32_t listed = *T.listCount;
   if (blockIdx.x == 0 && tid == 0) *T.epoch += 1u;  // the scan of this step is done
   for (uint32_t j = blockIdx.x; j < listed; j += gridDim.x) {
-    const uint32_t r = T.rowList[j];
-    const uint32_t o0 = T.startNext[r], m = T.startNext[r + 1] - o0;
-    const uint32_t a0 = T.startCur[r], A = T.startCur[r + 1] - a0;
-    const uint32_t c0 = T.chgStart[r * kRowCopies], C = T.chgStart[(r + 1) * kRowCopies] - c0;
+    if (j != blockIdx.x) {
+      e0 = T.rowList[2 * (size_t)j];
+      e1 = T.rowList[2 * (size_t)j + 1];
+    }
+    const uint32_t r = e0.x, o0 = e0.y, m = e0.z, a0 = e0.w;
+    const uint32_t A = e1.x, c0 = e1.y, C = e1.z;
     const uint32_t rowBase = r << T.shift;
     __syncthreads();  // the previous row's LDS reads are done
     if (m > kRowCap) {
       rank_by_counting(T, keys, idOfSlot, O, r, o0, m, a0, A, c0, C, s_key, s_wave, devFlags);
       continue;
     }
-    // gather: stayers (same cell) in order at the front, movers at the back;
-    // each thread takes a contiguous segment of the current range, counts
-    // its stayers, and after one scan places its members
+    // gather, kRowCap candidates at a time (one group for every row of at
+    // most kRowCap current spheres and incomers): candidate u = g0 + 128 t
+    // + tid, loads first; a stayer's place is the number of stayers before
+    // it in candidate order (wave ballots per 128-candidate chunk, counts
+    // exchanged in LDS); movers fill the pool from the back
     if (tid == 0) s_movers = 0u;
-    const uint32_t E = (A + kSortThreads - 1u) / kSortThreads;
-    const uint32_t u0 = min(A, tid * E), u1 = min(A, u0 + E);
-    uint32_t mine = 0;
-#pragma unroll 4
-    for (uint32_t u = u0; u < u1; ++u) {
-      const uint32_t raw = keys[a0 + u];
-      mine += (((raw & ~kMovedBit) >> T.shift) == r && (raw & kMovedBit) == 0u) ? 1u : 0u;
-    }
-    uint32_t stayers;  // the same in every thread
-    uint32_t at = block_exclusive_scan<kSortThreads>(mine, s_wave, &stayers);
-#pragma unroll 4
-    for (uint32_t u = u0; u < u1; ++u) {
-      const uint32_t k = a0 + u;
-      const uint32_t raw = keys[k], id = idOfSlot[k];
-      const uint32_t key = raw & ~kMovedBit;
-      if ((key >> T.shift) != r) continue;
-      const uint32_t pos =
-          (raw & kMovedBit) == 0u ? at++ : kRowCap - 1u - atomicAdd(&s_movers, 1u);
-      if (pos < kRowCap) {  // (beyond it the counts disagree: checked below)
-        s_key[pos] = member_key(key, T.shift, id);
-        s_slot[pos] = k;
+    const uint32_t U = A + C;
+    uint32_t stayers = 0;  // the same in every thread
+    for (uint32_t g0 = 0; g0 < U; g0 += kRowCap) {
+      uint32_t raw[kPerThread], idv[kPerThread], slot[kPerThread];
+#pragma unroll
+      for (uint32_t t = 0; t < kPerThread; ++t) {
+        const uint32_t u = g0 + t * kSortThreads + tid;
+        raw[t] = 0u;
+        idv[t] = 0u;
+        slot[t] = 0u;
+        if (u < A) {
+          slot[t] = a0 + u;
+          raw[t] = keys[a0 + u];
+          idv[t] = idOfSlot[a0 + u];
+        } else if (u < U) {
+          const uint4 c = T.chgList[c0 + (u - A)];
+          slot[t] = c.x;
+          raw[t] = c.y | kMovedBit;  // a sphere that changes row changes cell
+          idv[t] = c.z;
+        }
       }
-    }
-    for (uint32_t i = tid; i < C; i += kSortThreads) {
-      const uint32_t k = T.chgList[c0 + i];
-      const uint32_t pos = kRowCap - 1u - atomicAdd(&s_movers, 1u);
-      if (pos < kRowCap) {
-        s_key[pos] = member_key(keys[k] & ~kMovedBit, T.shift, idOfSlot[k]);
-        s_slot[pos] = k;
+      uint64_t stays[kPerThread];
+#pragma unroll
+      for (uint32_t t = 0; t < kPerThread; ++t) {
+        const uint32_t u = g0 + t * kSortThreads + tid;
+        const bool member = u < U && ((raw[t] & ~kMovedBit) >> T.shift) == r;
+        stays[t] = __ballot(member && (raw[t] & kMovedBit) == 0u);
+        if (lane == 0u) s_count[t][wave] = (uint32_t)__popcll(stays[t]);
       }
+      __syncthreads();
+      uint32_t before = stayers;  // stayers of the earlier chunks
+#pragma unroll
+      for (uint32_t t = 0; t < kPerThread; ++t) {
+        const uint32_t u = g0 + t * kSortThreads + tid;
+        const uint32_t w0 = s_count[t][0], w1 = s_count[t][1];
+        const uint32_t key = raw[t] & ~kMovedBit;
+        if (u < U && (key >> T.shift) == r) {
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(stays[t] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)stays[t], 0u));
+          const uint32_t pos = (raw[t] & kMovedBit) == 0u
+                                   ? before + (wave != 0u ? w0 : 0u) + below
+                                   : kRowCap - 1u - atomicAdd(&s_movers, 1u);
+          if (pos < kRowCap) {  // (beyond it the counts disagree: checked below)
+            s_key[pos] = member_key(key, T.shift, idv[t]);
+            s_slot[pos] = slot[t];
+          }
+        }
+        before += w0 + w1;
+      }
+      stayers = before;
+      __syncthreads();  // the pool is written; s_count may be reused
     }
-    __syncthreads();
+    if (U == 0u) {  // a listed row has members: the counts disagree
+      if (tid == 0) atomicOr(devFlags, DEVERR_ROW_COUNT);
+      continue;
+    }
     const uint32_t movers = s_movers;
     if (stayers + movers != m) {  // the counts and the members disagree: write nothing
       if (tid == 0) atomicOr(devFlags, DEVERR_ROW_COUNT);
@@ -579,25 +630,26 @@ fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n) {
   if (T.block != nullptr && T.rows == rows && T.shift == shift && T.capacity >= n)
     return FC_SUCCESS;
   free_row_tables(T);
-  // startCur, startNext [rows + 1]; chgStart [rows x copies + 1]; rowList
-  // [rows]; groupWords [3 groups, 64-bit]; listCount, epoch [2]; rowHist,
-  // chgHist, chgCursor [rows x copies] (zeroed); chgList [n]
-  // every array starts on a 64-word (256-byte) boundary
+  // count, startNext [rows + 1]; chgStart [rows x copies + 1]; rowList
+  // [2 rows, 16-byte]; groupWords [3 groups, 64-bit]; listCount, epoch [2];
+  // rowHist, chgHist, chgCursor [rows x copies] (zeroed); chgList [n, 16-byte]
+  // every array up to the counters starts on a 64-word (256-byte) boundary,
+  // chgList on a 32-byte one
   const size_t cells = (size_t)rows * kRowCopies;
   auto up = [](size_t x) { return (x + 63) & ~(size_t)63; };
   const size_t zeroWords = 3 * cells;
-  const size_t words = 2 * up(rows + 1) + up(cells + 1) + up(rows) + up(6 * (size_t)groups) +
-                       up(2) + zeroWords + (n ? n : 1);
+  const size_t words = 2 * up(rows + 1) + up(cells + 1) + up(8 * (size_t)rows) +
+                       up(6 * (size_t)groups) + up(2) + zeroWords + 4 * (size_t)(n ? n : 1);
   FC_HIP_TRY(hipMalloc(&T.block, words * sizeof(uint32_t)));
   uint32_t* w = T.block;
-  T.startCur = w;
+  T.count = w;
   w += up(rows + 1);
   T.startNext = w;
   w += up(rows + 1);
   T.chgStart = w;
   w += up(cells + 1);
-  T.rowList = w;
-  w += up(rows);
+  T.rowList = reinterpret_cast<uint4*>(w);
+  w += up(8 * (size_t)rows);
   T.groupWords = reinterpret_cast<uint64_t*>(w);
   w += up(6 * (size_t)groups);
   T.listCount = w;
@@ -611,7 +663,7 @@ fc_status ensure_row_tables(RowTables& T, const fc_sim_params& P, uint32_t n) {
   w += cells;
   T.chgCursor = w;
   w += cells;
-  T.chgList = w;
+  T.chgList = reinterpret_cast<uint4*>(w);
   T.rows = rows;
   T.shift = shift;
   T.groups = groups;
@@ -644,11 +696,12 @@ hipError_t launch_row_starts(hipStream_t s, const RowTables& T, const uint32_t* 
 }
 
 hipError_t launch_row_changers(hipStream_t s, const RowTables& T, const uint32_t* keys,
-                               const uint32_t* hash, uint32_t* cellStart, uint32_t* cellEnd,
-                               uint32_t n, uint32_t numCells, uint32_t* devFlags) {
+                               const uint32_t* idOfSlot, const uint32_t* hash,
+                               uint32_t* cellStart, uint32_t* cellEnd, uint32_t n,
+                               uint32_t numCells, uint32_t* devFlags) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_changers, dim3((n + 255u) / 256u), dim3(256), 0, s, T, keys, hash,
-                     cellStart, cellEnd, n, numCells, devFlags);
+  hipLaunchKernelGGL(k_row_changers, dim3((n + 255u) / 256u), dim3(256), 0, s, T, keys, idOfSlot,
+                     hash, cellStart, cellEnd, n, numCells, devFlags);
   return hipGetLastError();
 }
 
