@@ -465,12 +465,48 @@ def test_run_with_many_occupied_rows_matches_oracle(gpu_ctx):
     ps.close()
 
 
-@pytest.mark.parametrize("rows", [1, 8, 16])
-def test_run_with_crowded_rows_matches_oracle(gpu_ctx, rows):
+def _row_sort_cases(prev_hash, prev_index, hash_, index, grid, n):
+    """The row sort's cases in the resort from one sorted order to the next
+    (from the oracle's sorted hashes and ids): rows of more than 768
+    members (ranked by counting), rows of at most 768 members with more
+    than 256 movers (counting as well), rows whose current spheres and
+    incomers exceed 768 while at most 768 stay members (gathered in
+    several groups), and rows that empty."""
+    def per_id(h, idx):
+        out = np.empty(n, np.int64)
+        out[np.asarray(idx, np.int64)] = np.asarray(h, np.int64)
+        return out
+    cell0, cell1 = per_id(prev_hash, prev_index), per_id(hash_, index)
+    row0, row1 = cell0 // grid, cell1 // grid
+    nr = grid * grid
+    A = np.bincount(row0, minlength=nr)
+    m = np.bincount(row1, minlength=nr)
+    C = np.bincount(row1[row0 != row1], minlength=nr)
+    stay = np.bincount(row1[cell0 == cell1], minlength=nr)
+    cases = set()
+    if np.any(m > 768):
+        cases.add("counting, size")
+    if np.any((m <= 768) & (m - stay > 256)):
+        cases.add("counting, movers")
+    if np.any((A + C > 768) & (m <= 768) & (m > 0)):
+        cases.add("gather in groups")
+    if np.any((m == 0) & (A > 0)):
+        cases.add("row empties")
+    return cases
+
+
+@pytest.mark.parametrize("rows,cases", [
+    (1, {"counting, size", "counting, movers", "gather in groups", "row empties"}),
+    (8, {"counting, movers", "gather in groups", "row empties"}),
+    (16, {"gather in groups", "row empties"}),
+])
+def test_run_with_crowded_rows_matches_oracle(gpu_ctx, rows, cases):
     # 20,000 spheres packed into `rows` grid rows of a 32^3 grid (x spread
-    # over the row, y and z inside one cell per row): 20000, 2500 and 1250
-    # spheres per row, above and below what a row sorts in LDS; the packed
-    # spheres push apart, so many change row every step.
+    # over the row and wrapping round it, y and z inside one cell per row):
+    # 20000, 2500 and 1250 spheres per row, far above what a row sorts in
+    # LDS; the packed spheres push apart, so many change row every step.
+    # The oracle's trajectory shows which of the row sort's cases the run's
+    # four resorted steps reach.
     n, grid, r = 20_000, 32, 1.0 / 64.0
     print(f"[crowded] n={n} rows={rows}", flush=True)
     from fullc0ntact_amd import ParticleSystem
@@ -478,21 +514,28 @@ def test_run_with_crowded_rows_matches_oracle(gpu_ctx, rows):
     P = sim_params(n, grid, r)
     ps.m_params = P
     rng = np.random.default_rng(31 + rows)
-    cell = 2.0 / grid
+    cell = float(P.cellSize.x)
     pos = np.zeros((n, 4), np.float32)
     pos[:, 3] = 1.0
     which = rng.integers(0, rows, n)
     pos[:, 0] = rng.uniform(-0.99, 0.99, n)
-    pos[:, 1] = (which % 4) * 4 * cell + rng.uniform(0.1, 0.9, n) * cell - 0.5
-    pos[:, 2] = (which // 4) * 4 * cell + rng.uniform(0.1, 0.9, n) * cell - 0.5
+    pos[:, 1] = -1.0 + (4 * (which % 4) + 8 + rng.uniform(0.1, 0.9, n)) * cell
+    pos[:, 2] = -1.0 + (4 * (which // 4) + 8 + rng.uniform(0.1, 0.9, n)) * cell
     vel = np.zeros((n, 4), np.float32)
     ps.setArray(ps.POSITION, pos, 0, n)
     ps.setArray(ps.VELOCITY, vel, 0, n)
     st = orc.State(pos, vel, P.numCells)
     OP = orc.params_from(P)
     ps.run(0.5, 5)
+    reached, prev = set(), None
     for _ in range(5):
         st.update(0.5, OP)
+        cur = (np.array(st.hash), np.array(st.index))
+        if prev is not None:
+            reached |= _row_sort_cases(*prev, *cur, grid, n)
+        prev = cur
+    print(f"[crowded] rows={rows} row sort cases: {sorted(reached)}", flush=True)
+    assert reached == cases
     _compare_system(ps, st)
     ps.close()
 
