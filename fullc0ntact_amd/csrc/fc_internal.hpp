@@ -161,14 +161,16 @@ hipError_t radix_sort_pairs(hipStream_t stream, SortScratch& s, uint32_t* devFla
                             StageTimer* timer, bool histogramReady = false, int minPasses = 1,
                             const uint32_t* clearHash = nullptr, uint32_t* cellStart = nullptr,
                             uint32_t* cellEnd = nullptr, const SortGather* gather = nullptr);
-// Sort of n <= RS_TILE pairs in one single-block launch that also does the
-// cell-table work of a system step: resets the cells of clearHash, gathers
-// the state (gather), and writes cellStart/cellEnd of the sorted keys.
-// Bit-identical to radix_sort_pairs (with the same clearHash and gather)
-// followed by launch_cell_bounds, for keys below numCells (the cells of a
-// system); it sorts on the keys' width, at most 8 * passes bits.  The
-// digit-0 histogram of keysIn is not read; H[0] is zeroed for the next
-// sort.
+// Sort of n <= RS_TILE pairs by counting, in two launches that also do the
+// cell-table work of a system step: every key's rank in the stable order
+// (one wave per key over all keys staged in LDS) into keysA/valsA of the
+// scratch, with the cells of clearHash reset; then the sorted keys and
+// values, the gathered state (gather) and cellStart/cellEnd of the sorted
+// keys.  Bit-identical to radix_sort_pairs (with the same clearHash and
+// gather) followed by launch_cell_bounds, for keys below numCells (the
+// cells of a system, at most 8 * passes bits wide); inputs and outputs may
+// alias.  The digit-0 histogram of keysIn is not read; H[0] is zeroed for
+// the next sort.
 hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t* keysIn,
                              const uint32_t* valsIn, uint32_t* keysOut, uint32_t* valsOut,
                              uint32_t n, int passes, const uint32_t* clearHash,

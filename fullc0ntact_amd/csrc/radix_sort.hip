@@ -380,156 +380,94 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   if (countNext) histogram_flush(s_nextHist, gHist, pass + 1);
 }
 
-// The whole sort of n <= RS_TILE keys in one block, for small systems whose
-// step would otherwise be four launches of one block each: every pass ranks
-// the keys in LDS as k_rs_scatter ranks a tile (wave ballots, wave-private
-// digit counts, block scan), which in a single tile is the global rank, so
-// no histogram, look-back or further launch is needed; the digits are 9
-// bits wide (two passes for the 18-bit cells of a 64^3 grid instead of
-// three).
-// The block also does the other work of those launches: it resets the
-// cells of the previous sorted hashes (clearHash) and zeroes H[0] for the
-// next sort (as pass 1 does), gathers the state (SortGather) and writes the
-// cell bounds of the sorted keys (k_cell_bounds).  Outputs are bit-identical
-// to radix_sort_pairs + launch_cell_bounds.
-// The block holds every key, so its digits may be wider than the
-// multi-block passes' (kSmallBits): fewer passes over the same key width.
-constexpr int kSmallBits = 9;
-constexpr int kSmallRadix = 1 << kSmallBits;
-static_assert(kSmallRadix <= RS_THREADS && kSmallRadix % 64 == 0, "one digit per thread");
+// The sort of n <= RS_TILE keys by counting, over many blocks: a key's place
+// in the stable order is the number of (key, index) pairs below its own,
+// so every block stages all n pairs in LDS and counts for 16 of them, a
+// wave per key (lane s compares against the pairs j = s mod 64, so that
+// a wave reads 64 consecutive words), with no exchange between blocks.  C packs (key << 12) | index, 32 bits wide when the keys' width
+// leaves 12 bits for the index, else 64.  The block also resets the cells
+// of its 16 entries of the previous sorted hashes (clearHash), which no
+// later write of this launch touches, and block 0 zeroes H[0] for the next
+// sort.  The sorted pairs go to keysStage/valsStage; k_small_emit writes
+// the outputs in a second launch, so that inputs and outputs may alias.
+constexpr uint32_t kRankLanes = 64;                  // lanes per key: one wave
+constexpr uint32_t kRankThreads = 1024;
+constexpr uint32_t kRankKeys = kRankThreads / kRankLanes;  // keys per block
+static_assert(RS_TILE % kRankLanes == 0 && (1u << 12) >= (uint32_t)RS_TILE, "12-bit index");
 
-__global__ __launch_bounds__(RS_THREADS) void k_sort_small(
-    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
-    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n, int passes,
+template <typename C>
+__global__ __launch_bounds__(kRankThreads) void k_small_rank(
+    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn, uint32_t n,
     const uint32_t* __restrict__ clearHash, uint32_t* __restrict__ cellStart,
-    uint32_t* __restrict__ cellEnd, uint32_t numCells, SortGather ga, uint32_t* gHist) {
-  __shared__ uint32_t s_keys[RS_TILE];
-  __shared__ uint32_t s_vals[RS_TILE];
-  __shared__ uint32_t s_waveHist[kWaves][kSmallRadix];
-  __shared__ uint32_t s_tileStart[kSmallRadix];
-  __shared__ uint32_t s_scratch[kSmallRadix / 64];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t waveBase = wave * kWaveKeys;
-  {
+    uint32_t* __restrict__ cellEnd, uint32_t* __restrict__ keysStage,
+    uint32_t* __restrict__ valsStage, uint32_t* gHist) {
+  __shared__ C s_pair[RS_TILE];
+  const uint32_t tid = threadIdx.x;
+  if (blockIdx.x == 0) {
     uint32_t* h0 = hist_buffer(gHist, 0);
-    for (int i = threadIdx.x; i < RS_HIST_COPIES * RS_RADIX; i += RS_THREADS) h0[i] = 0u;
+    for (uint32_t i = tid; i < RS_HIST_COPIES * RS_RADIX; i += kRankThreads) h0[i] = 0u;
   }
-  if (clearHash != nullptr) {
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-      const uint32_t idx = i * RS_THREADS + threadIdx.x;
-      if (idx < n) {
-        const uint32_t h = clearHash[idx];
-        if (idx == 0 || clearHash[idx - 1] != h) {
-          cellStart[h] = CELL_EMPTY;
-          cellEnd[h] = CELL_EMPTY;
-        }
+  if (clearHash != nullptr && tid < kRankKeys) {
+    const uint32_t idx = blockIdx.x * kRankKeys + tid;
+    if (idx < n) {
+      const uint32_t h = clearHash[idx];
+      if (idx == 0 || clearHash[idx - 1] != h) {
+        cellStart[h] = CELL_EMPTY;
+        cellEnd[h] = CELL_EMPTY;
       }
     }
   }
-  uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+  const uint32_t padded = (n + kRankLanes - 1u) & ~(kRankLanes - 1u);
+  for (uint32_t j = tid; j < padded; j += kRankThreads)
+    s_pair[j] = j < n ? (((C)keysIn[j] << 12) | (C)j) : ~(C)0;
+  const uint32_t seg = tid & (kRankLanes - 1u);
+  const uint32_t i = blockIdx.x * kRankKeys + tid / kRankLanes;
+  const uint32_t val = (seg == 0u && i < n) ? (valsIn ? valsIn[i] : i) : 0u;
+  __syncthreads();
+  const C mine = i < n ? s_pair[i] : ~(C)0;
+  uint32_t below = 0;
+#pragma unroll 8
+  for (uint32_t j = seg; j < padded; j += kRankLanes) below += s_pair[j] < mine ? 1u : 0u;
 #pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
-    const uint32_t idx = waveBase + i * 64u + lane;
-    const bool valid = idx < n;
-    key[i] = valid ? keysIn[idx] : 0xffffffffu;
-    val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
+  for (uint32_t d = kRankLanes / 2; d > 0; d >>= 1) below += __shfl_xor(below, d, kRankLanes);
+  if (seg == 0u && i < n) {
+    keysStage[below] = (uint32_t)(mine >> 12);
+    valsStage[below] = val;
   }
-  for (int pass = 0; pass < passes; ++pass) {
-    const uint32_t shift = (uint32_t)pass * kSmallBits;
-    for (int i = threadIdx.x; i < kWaves * kSmallRadix; i += RS_THREADS) (&s_waveHist[0][0])[i] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-      const uint32_t idx = waveBase + i * 64u + lane;
-      const bool valid = idx < n;
-      const uint64_t active = __ballot(valid);
-      const uint32_t d = (key[i] >> shift) & (kSmallRadix - 1);
-      const uint64_t peers = match_digit<kSmallBits>(d, active);
-      const uint32_t lower = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-      const uint32_t before = s_waveHist[wave][d];
-      rank[i] = before + lower;
-      if (valid && lower == 0u) s_waveHist[wave][d] = before + (uint32_t)__popcll(peers);
-    }
-    __syncthreads();
-    const uint32_t d = threadIdx.x;
-    const bool digitThread = d < (uint32_t)kSmallRadix;
-    uint32_t count = 0;
-    if (digitThread) {
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        const uint32_t c = s_waveHist[w][d];
-        s_waveHist[w][d] = count;
-        count += c;
-      }
-    }
-    const uint32_t start = scan_exclusive<kSmallRadix>(digitThread ? count : 0u, s_scratch);
-    if (digitThread) s_tileStart[d] = start;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-      const uint32_t idx = waveBase + i * 64u + lane;
-      if (idx < n) {
-        const uint32_t dg = (key[i] >> shift) & (kSmallRadix - 1);
-        const uint32_t lp = s_tileStart[dg] + s_waveHist[wave][dg] + rank[i];
-        s_keys[lp] = key[i];
-        s_vals[lp] = val[i];
-      }
-    }
-    __syncthreads();
-    if (pass + 1 < passes) {
-#pragma unroll
-      for (int i = 0; i < RS_ITEMS; ++i) {
-        const uint32_t idx = waveBase + i * 64u + lane;
-        key[i] = idx < n ? s_keys[idx] : 0xffffffffu;
-        val[i] = idx < n ? s_vals[idx] : 0u;
-      }
-      __syncthreads();
-    }
+}
+
+// The outputs of the sort by counting, one thread per sorted position:
+// keys and values, the gathered state, and the cell bounds (as
+// k_cell_bounds writes them).
+__global__ __launch_bounds__(64) void k_small_emit(
+    const uint32_t* __restrict__ keysStage, const uint32_t* __restrict__ valsStage,
+    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut, uint32_t n,
+    uint32_t* __restrict__ cellStart, uint32_t* __restrict__ cellEnd, uint32_t numCells,
+    SortGather ga) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t k = keysStage[g], v = valsStage[g];
+  const uint32_t kp = g > 0 ? keysStage[g - 1] : 0u;
+  float4 pv = make_float4(0.f, 0.f, 0.f, 0.f), vv = pv;
+  uint32_t id = 0;
+  if (ga.pos && v < n) {
+    pv = ga.pos[v];
+    vv = ga.vel[v];
+    if (ga.idOfSlot) id = ga.idOfSlot[v];
   }
-  // Sorted keys and values are in s_keys/s_vals[0, n); the clears above
-  // completed before the first barrier, so the bounds below overwrite them.
-  // All items' state loads are issued before any store (see k_rs_scatter).
-  uint32_t ks[RS_ITEMS], vs[RS_ITEMS], kps[RS_ITEMS], id[RS_ITEMS];
-  float4 pv[RS_ITEMS], vv[RS_ITEMS];
-#pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
-    const uint32_t g = i * RS_THREADS + threadIdx.x;
-    ks[i] = g < n ? s_keys[g] : 0u;
-    vs[i] = g < n ? s_vals[g] : 0u;
-    kps[i] = (g > 0 && g < n) ? s_keys[g - 1] : 0u;
+  keysOut[g] = k;
+  valsOut[g] = v;
+  if (ga.pos && v < n) {
+    ga.sortedPos[g] = pv;
+    ga.sortedVel[g] = make_float4(vv.x, vv.y, vv.z, __uint_as_float(k));
+    if (ga.idOfSlot) ga.indexOut[g] = id;
+    if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv.x, pv.y, pv.z);
   }
-  if (ga.pos) {
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-      const uint32_t g = i * RS_THREADS + threadIdx.x;
-      const uint32_t v = (g < n && vs[i] < n) ? vs[i] : 0u;
-      pv[i] = ga.pos[v];
-      vv[i] = ga.vel[v];
-      id[i] = ga.idOfSlot ? ga.idOfSlot[v] : 0u;
-    }
+  if (g == 0 || k != kp) {
+    if (k < numCells) cellStart[k] = g;
+    if (g > 0 && kp < numCells) cellEnd[kp] = g;
   }
-#pragma unroll
-  for (int i = 0; i < RS_ITEMS; ++i) {
-    const uint32_t g = i * RS_THREADS + threadIdx.x;
-    if (g < n) {
-      const uint32_t k = ks[i], v = vs[i];
-      keysOut[g] = k;
-      valsOut[g] = v;
-      if (ga.pos && v < n) {
-        ga.sortedPos[g] = pv[i];
-        ga.sortedVel[g] = make_float4(vv[i].x, vv[i].y, vv[i].z, __uint_as_float(k));
-        if (ga.idOfSlot) ga.indexOut[g] = id[i];
-        if (ga.sortedQ) ga.sortedQ[g] = quantize_pos(ga.frame, pv[i].x, pv[i].y, pv[i].z);
-      }
-      const uint32_t kp = kps[i];
-      if (g == 0 || k != kp) {
-        if (k < numCells) cellStart[k] = g;
-        if (g > 0 && kp < numCells) cellEnd[kp] = g;
-      }
-      if (g == n - 1 && k < numCells) cellEnd[k] = g + 1;
-    }
-  }
+  if (g == n - 1 && k < numCells) cellEnd[k] = g + 1;
 }
 
 }  // namespace
@@ -541,14 +479,21 @@ hipError_t launch_sort_small(hipStream_t stream, SortScratch& s, const uint32_t*
                              const SortGather& gather) {
   if (n == 0) return hipSuccess;
   if (n > (uint32_t)RS_TILE || passes < 1 || passes > RS_MAX_PASSES) return hipErrorInvalidValue;
-  // The keys are cells, below numCells: their width in kSmallBits-bit
-  // digits (at least one pass, so that the sorted tile is staged).
+  // The keys are cells, below numCells: (key << 12) | index fits 32 bits
+  // for keys of up to 20 bits.
   int bits = key_bits_for_cells(numCells);
   if (bits > passes * RS_RADIX_BITS) bits = passes * RS_RADIX_BITS;
-  const int smallPasses = bits > kSmallBits ? (bits + kSmallBits - 1) / kSmallBits : 1;
-  hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(RS_THREADS), 0, stream, keysIn, valsIn, keysOut,
-                     valsOut, n, smallPasses, clearHash, cellStart, cellEnd, numCells, gather,
-                     s.histogram);
+  const dim3 rankGrid((n + kRankKeys - 1) / kRankKeys);
+  if (bits <= 20)
+    hipLaunchKernelGGL(k_small_rank<uint32_t>, rankGrid, dim3(kRankThreads), 0, stream, keysIn,
+                       valsIn, n, clearHash, cellStart, cellEnd, s.keysA, s.valsA, s.histogram);
+  else
+    hipLaunchKernelGGL(k_small_rank<uint64_t>, rankGrid, dim3(kRankThreads), 0, stream, keysIn,
+                       valsIn, n, clearHash, cellStart, cellEnd, s.keysA, s.valsA, s.histogram);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_small_emit, dim3((n + 63) / 64), dim3(64), 0, stream, s.keysA, s.valsA,
+                     keysOut, valsOut, n, cellStart, cellEnd, numCells, gather);
   return hipGetLastError();
 }
 
