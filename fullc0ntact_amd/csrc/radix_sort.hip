@@ -46,7 +46,6 @@ namespace {
 constexpr int kWaves = RS_THREADS / 64;
 // Sorts of at least this many keys use 8192-key tiles (k_rs_scatter).
 constexpr uint32_t kWideTileKeys = 1u << 22;
-constexpr int kWaveKeys = 64 * RS_ITEMS;
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPrefix = 2ull << 62;
 constexpr uint64_t kFlagMask = 3ull << 62;
