@@ -554,6 +554,9 @@ def main(argv=None) -> int:
                     help="leg whose measured region is bracketed for rocprofv3 --selected-regions")
     ap.add_argument("--progress-every", type=int, default=0,
                     help="run the warmup in chunks of this many steps, logging each")
+    ap.add_argument("--small-case-only", action="store_true",
+                    help="run only the 4096-sphere case (GPU and CPU) and print its result, "
+                         "for profiling that case on its own")
     a = ap.parse_args(argv)
     global PROFILE_LEG
     PROFILE_LEG = a.profile_leg
@@ -561,6 +564,9 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported "
               "(this build has no multi-GPU path).", file=sys.stderr)
         return 2
+    if a.small_case_only:
+        print(json.dumps({"small_case": small_case_leg(host_threads()[0])}), flush=True)
+        return 0
 
     g = gpu_leg(a.steps, a.warmup, a.timing_steps, a.settle, a.progress_every)
     ms = g["elapsed"] / a.steps * 1e3
