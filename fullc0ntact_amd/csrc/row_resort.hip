@@ -35,10 +35,11 @@ namespace fc {
 namespace {
 
 constexpr int kScanThreads = 256;
-// The row sort: one row per 128-thread block, 9 KB of LDS and at most 64
-// VGPRs, so that 16 blocks fit a CU and the ~3800 occupied rows of a
-// 1M-sphere bed are all in flight at once (each row is a chain of dependent
-// loads).
+// The row sort: one row per 128-thread block (9 KB of LDS, 82 VGPRs: 10
+// blocks per CU) over a fixed grid of 8192 blocks, so that the ~3800
+// occupied rows of a 1M-sphere bed are in flight together (each row is a
+// chain of dependent loads; capping the kernel at 64 or 72 VGPRs for more
+// blocks per CU made it slower).
 constexpr int kSortThreads = 128;
 constexpr uint32_t kRowCap = 768;      // members of a row held in LDS
 constexpr uint32_t kMoverCap = 256;    // movers ranked by counting
