@@ -438,8 +438,7 @@ fc_status enqueue_batch_update(fc_batch* b, float dt) {
   if (b->tablesTrackHash) {
     prev = b->hash;
   } else {
-    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells));
-    FC_HIP_TRY(launch_fill_u32(st, b->cellEnd, CELL_EMPTY, cells));
+    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells, b->cellEnd));
   }
   if (b->n == 0) return FC_SUCCESS;
   const int bits = batch_key_bits(b);
@@ -626,8 +625,7 @@ fc_status enqueue_batch_run(fc_batch* b, float dt, uint32_t steps) {
   if (b->tablesTrackHash) {
     prev = b->hash;
   } else {
-    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells));
-    FC_HIP_TRY(launch_fill_u32(st, b->cellEnd, CELL_EMPTY, cells));
+    FC_HIP_TRY(launch_fill_u32(st, b->cellStart, CELL_EMPTY, cells, b->cellEnd));
   }
   // the row counts start from zero (a run that stopped early may have left
   // counts behind)

@@ -26,8 +26,11 @@ inline unsigned grid_for(size_t n, int block) {
   return (unsigned)(g == 0 ? 1 : g);
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t value,
+// Fills p (blockIdx.y 0) and p2 (blockIdx.y 1) with value.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p0,
+                                                     uint32_t* __restrict__ p2, uint32_t value,
                                                      size_t count) {
+  uint32_t* __restrict__ p = blockIdx.y == 0 ? p0 : p2;
   size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x;
   size_t stride = (size_t)gridDim.x * kBlock;
   // 16-byte stores over the aligned body, scalar stores for the ragged ends.
@@ -471,12 +474,13 @@ __global__ __launch_bounds__(kBlock) void k_collide_split(
 
 }  // namespace
 
-hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count) {
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count,
+                           uint32_t* p2) {
   if (count == 0) return hipSuccess;
   size_t vec = count / 4 + 2;
   unsigned grid = grid_for(vec, kBlock);
   if (grid > 2048u) grid = 2048u;
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(kBlock), 0, s, p, value, count);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid, p2 ? 2 : 1), dim3(kBlock), 0, s, p, p2, value, count);
   return hipGetLastError();
 }
 

@@ -445,8 +445,7 @@ fc_status fc_reorder_data_and_find_cell_start(fc_context* ctx, uint32_t* cellSta
     return set_error(FC_ERROR_INVALID_VALUE, "NULL cell table");
   StageTimer* t = stage_timer(ctx);
   if (t) t->begin(ctx->stream, "drop_in_clear_tables");
-  hipError_t e = launch_fill_u32(ctx->stream, cellStart, CELL_EMPTY, numCells);
-  if (e == hipSuccess) e = launch_fill_u32(ctx->stream, cellEnd, CELL_EMPTY, numCells);
+  hipError_t e = launch_fill_u32(ctx->stream, cellStart, CELL_EMPTY, numCells, cellEnd);
   if (t) t->end(ctx->stream);
   if (e != hipSuccess) return hip_error(e, "clear cell tables");
   if (numParticles == 0) return FC_SUCCESS;

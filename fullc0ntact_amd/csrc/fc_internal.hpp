@@ -209,7 +209,9 @@ struct StageTimer {
 // DEM stage launchers (dem_kernels.hip).  Parameters are passed by value as
 // kernel arguments (no constant-memory upload per step).
 // ---------------------------------------------------------------------------
-hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
+// Fills count words of p, and of p2 when given, with value (one launch).
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count,
+                           uint32_t* p2 = nullptr);
 hipError_t launch_copy16(hipStream_t s, void* dst, const void* src, size_t bytes);
 hipError_t launch_integrate(hipStream_t s, const fc_sim_params& P, float4* pos, float4* vel,
                             float dt, uint32_t n);

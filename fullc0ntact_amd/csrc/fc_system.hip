@@ -95,8 +95,7 @@ fc_status enqueue_update(fc_system* sys, float dt, StageTimer* timer) {
     prev = sys->hash;
   } else {
     if (timer) timer->begin(st, "clear_tables");
-    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells);
-    if (e == hipSuccess) e = launch_fill_u32(st, sys->cellEnd, CELL_EMPTY, P.numCells);
+    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells, sys->cellEnd);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "clear cell tables");
   }
@@ -190,8 +189,7 @@ fc_status enqueue_run_head(fc_system* sys, float dt, StageTimer* timer) {
     prev = sys->hash;
   } else {
     if (timer) timer->begin(st, "clear_tables");
-    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells);
-    if (e == hipSuccess) e = launch_fill_u32(st, sys->cellEnd, CELL_EMPTY, P.numCells);
+    e = launch_fill_u32(st, sys->cellStart, CELL_EMPTY, P.numCells, sys->cellEnd);
     if (timer) timer->end(st);
     if (e != hipSuccess) return hip_error(e, "clear cell tables");
   }
