@@ -426,6 +426,52 @@ def sdf_leg(reps: int) -> dict:
 
 
 BATCH_SCENES, BATCH_LO, BATCH_HI, BATCH_GRID = 256, 2000, 200000, 64
+HOPPER_N, HOPPER_GRID, HOPPER_MEDIAN = 1 << 22, 512, 1.0 / 512
+
+
+def hopper_leg(settle: int, steps: int) -> dict:
+    """The polydisperse hopper: 4,194,304 spheres with log-normal radii
+    (median 1/512, sigma 0.25, clipped to a 3x ratio) on the jittered
+    lattice, the cell the largest diameter (a 512^3 grid), the default
+    parameters, and a (2, 3) torus-knot tube of 65,536 triangles as the
+    rigid boundary through an FP32 256^3 distance map.  `settle` untimed
+    steps of the fall, then a graph-launched run of `steps` steps timed.
+    Only the early fall is timed: at this radius the reference's spring
+    cannot hold the spheres apart at dt 0.5, and within a few hundred steps
+    the pile collapses to over a thousand spheres per cell, whose cost per
+    step is the collapse's, not the step's."""
+    from fullc0ntact_amd import ParticleSystem, make_params, meshes, statistics
+    from fullc0ntact_amd.distance_map import DistanceMap
+
+    rmax = float(np.float32(HOPPER_MEDIAN * np.sqrt(3.0)))
+    ps = ParticleSystem(HOPPER_N, (HOPPER_GRID,) * 3)
+    t0 = time.perf_counter()
+    v, f = meshes.torus_knot(2, 3, 1024, 32, radius=0.12, scale=0.25, center=(0.0, 0.0, -0.55))
+    gm = DistanceMap.from_mesh(ps.ctx, v, f, (256,) * 3, 2.0 / 256, (-1.0, -1.0, -1.0))
+    map_s = time.perf_counter() - t0
+    ps.m_params = make_params(HOPPER_N, HOPPER_GRID, rmax)
+    ps.reset(ps.CONFIG_GRID, seed=SEED, jitter_frac=JITTER, vel_amp=VAMP)
+    pos = ps.getArray(ps.POSITION).copy()
+    pos[:, 3] = statistics.polydisperse_radii(HOPPER_N, HOPPER_MEDIAN, 0.25, 3.0, seed=5)
+    ps.setArray(ps.POSITION, pos, 0, HOPPER_N)
+    ps.setPolydisperse(True)
+    ps.setBoundary(gm)
+    ps.run(DT, settle)
+    ps.sync()
+    t0 = time.perf_counter()
+    ps.run(DT, steps)
+    ps.sync()
+    el = time.perf_counter() - t0
+    ps.setBoundary(None)
+    ps.close()
+    gm.close()
+    log(f"[bench] hopper: {HOPPER_N * steps / el:.3e} particle-steps/s "
+        f"({el / steps * 1e3:.3f} ms per step; map built in {map_s:.1f} s)")
+    return {"particles": HOPPER_N, "grid": [HOPPER_GRID] * 3, "radius_median": HOPPER_MEDIAN,
+            "triangles": int(len(f)), "map_cells": [256] * 3, "timed_from_step": settle,
+            "steps": steps, "value": round(HOPPER_N * steps / el, 1),
+            "unit": "particle-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+            "map_build_s": round(map_s, 2)}
 
 
 def batch_leg(steps: int) -> dict:
@@ -550,6 +596,9 @@ def main(argv=None) -> int:
     ap.add_argument("--sdf-reps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-hopper", action="store_true", help="skip the polydisperse hopper leg")
+    ap.add_argument("--hopper-settle", type=int, default=10)
+    ap.add_argument("--hopper-steps", type=int, default=20)
     ap.add_argument("--profile-leg", choices=["dem", "sdf", "batch"], default="dem",
                     help="leg whose measured region is bracketed for rocprofv3 --selected-regions")
     ap.add_argument("--progress-every", type=int, default=0,
@@ -607,6 +656,9 @@ def main(argv=None) -> int:
     batch = None
     if not a.no_batch:
         batch = batch_leg(a.batch_steps)
+    hopper = None
+    if not a.no_hopper:
+        hopper = hopper_leg(a.hopper_settle, a.hopper_steps)
 
     cpu = None
     if not a.no_cpu:
@@ -653,6 +705,7 @@ def main(argv=None) -> int:
         "drop_in_per_stage_path": g["drop_in"],
         "sdf_vs_distance_map": sdf,
         "batched_ensemble": batch,
+        "polydisperse_hopper": hopper,
     }
     print(json.dumps(line), flush=True)
     return 0
