@@ -427,6 +427,71 @@ struct fc_distmap {
 
 namespace fc {
 
+namespace {
+// Smallest and largest vertex distance of each block of 8^3 cells (its
+// 9^3 vertices); a NaN distance makes the block's bounds NaN.
+__global__ __launch_bounds__(kBlock) void k_distmap_block_bounds(DistMapView m,
+                                                                 float2* __restrict__ out,
+                                                                 int bz) {
+  const int nb = m.bx * m.by * bz;
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nb) return;
+  const int ix = c % m.bx, iy = (c / m.bx) % m.by, iz = c / (m.bx * m.by);
+  const int x0 = ix << kMapBlockShift, y0 = iy << kMapBlockShift, z0 = iz << kMapBlockShift;
+  const int x1 = min(x0 + (1 << kMapBlockShift), m.cx), y1 = min(y0 + (1 << kMapBlockShift), m.cy),
+            z1 = min(z0 + (1 << kMapBlockShift), m.cz);
+  const int64_t sx = m.cx + 1, sxy = (int64_t)(m.cx + 1) * (m.cy + 1);
+  float lo = INFINITY, hi = -INFINITY;
+  bool nan = false;
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        const float d = m.cpd[z * sxy + y * sx + x].w;
+        nan = nan || d != d;
+        lo = fminf(lo, d);
+        hi = fmaxf(hi, d);
+      }
+  out[c] = nan ? make_float2(NAN, NAN) : make_float2(lo, hi);
+}
+}  // namespace
+
+fc_status distmap_block_bounds(const fc_distmap* m, hipStream_t s, float2** out, int* bx,
+                               int* by) {
+  *out = nullptr;
+  *bx = *by = 0;
+  // only on the uniform lattice bmin + i cellSize (see distmap_far)
+  const float* coords[3] = {m->xs, m->ys, m->zs};
+  for (int a = 0; a < 3; ++a) {
+    std::vector<float> c((size_t)m->cells[a] + 1);
+    FC_HIP_TRY(hipMemcpy(c.data(), coords[a], c.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c.size(); ++i) {
+      const double want = (double)m->bmin[a] + (double)i * (double)m->cellSize;
+      if (!(std::fabs((double)c[i] - want) <= 1e-5 * (double)m->cellSize)) return FC_SUCCESS;
+    }
+  }
+  const int nbx = (m->cells[0] + (1 << kMapBlockShift) - 1) >> kMapBlockShift;
+  const int nby = (m->cells[1] + (1 << kMapBlockShift) - 1) >> kMapBlockShift;
+  const int nbz = (m->cells[2] + (1 << kMapBlockShift) - 1) >> kMapBlockShift;
+  const size_t nb = (size_t)nbx * nby * nbz;
+  float2* d = nullptr;
+  FC_HIP_TRY(hipMalloc(&d, nb * sizeof(float2)));
+  DistMapView v = distmap_view(m);
+  v.bx = nbx;
+  v.by = nby;
+  hipLaunchKernelGGL(k_distmap_block_bounds, dim3((unsigned)((nb + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, v, d, nbz);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_error(e, "distance-map block bounds");
+  }
+  *out = d;
+  *bx = nbx;
+  *by = nby;
+  return FC_SUCCESS;
+}
+
 DistMapView distmap_view(const fc_distmap* m) {
   DistMapView v;
   v.cx = m->cells[0];

@@ -26,7 +26,13 @@ struct DistMapView {
   const float* zs;
   const float4* cpd;
   const float4* nrm;
+  // Optional: per block of 8^3 cells, the smallest and largest distance of
+  // its 9^3 vertices (distmap_far), with the blocks per x and y row.
+  const float2* blockBounds = nullptr;
+  int bx = 0, by = 0;
 };
+
+constexpr int kMapBlockShift = 3;  // 8 cells per block edge
 
 __device__ __forceinline__ int dm_clamp_cell(int c, int cells) {
   // q on the upper face gives c == cells, whose +1 corner lies outside the
@@ -90,6 +96,32 @@ __device__ __forceinline__ v3f normalize_ref(v3f v) {
   return make_v3(v.x * inv, v.y * inv, v.z * inv);
 }
 
+// Whether the map distance at q is certainly at least r, so that queryMap
+// there would make no contact (it tests dist < r).  The query takes the
+// cell of q and mixes its eight corner distances with weights from q's
+// place in the cell: a convex mix, never below the smallest corner, except
+// that rounding can put q a few 10^-5 cells outside its cell and the mix
+// then extrapolates by that much times the corners' spread (on a lattice
+// whose coordinates are the uniform ones; the caller supplies blockBounds
+// only for such maps).  So a block whose smallest vertex distance m
+// satisfies m >= r + 10^-3 (max - min) + 10^-6 |m| is clear of contact for
+// every q in it.  False outside the bounding box (the query's own 5.0
+// sentinel applies there) and whenever the bounds are absent or NaN.
+__device__ __forceinline__ bool distmap_far(const DistMapView& m, v3f q, float r) {
+  if (m.blockBounds == nullptr) return false;
+  if (!(m.bmin[0] <= q.x && q.x <= m.bmax[0] && m.bmin[1] <= q.y && q.y <= m.bmax[1] &&
+        m.bmin[2] <= q.z && q.z <= m.bmax[2]))
+    return false;
+  const float inv = (float)(1.0 / (double)m.cellSize);
+  const int x = dm_clamp_cell((int)(fabsf(q.x - m.bmin[0]) * inv), m.cx);
+  const int y = dm_clamp_cell((int)(fabsf(q.y - m.bmin[1]) * inv), m.cy);
+  const int z = dm_clamp_cell((int)(fabsf(q.z - m.bmin[2]) * inv), m.cz);
+  const float2 b = m.blockBounds[((size_t)(z >> kMapBlockShift) * m.by + (y >> kMapBlockShift)) *
+                                     m.bx +
+                                 (x >> kMapBlockShift)];
+  return b.x >= r + 1e-3f * (b.y - b.x) + 1e-6f * fabsf(b.x);
+}
+
 // d_integrateSystem (particledem_um.cu:604-638) with d_checkBoundaryDistMap
 // and checkDistMap (boundaries.cuh:38-89): semi-implicit Euler, push-out of
 // the distance-map boundary, a cylinder of radius 1 around the z axis, and
@@ -104,8 +136,9 @@ __device__ __forceinline__ void integrate_distmap(const fc_sim_params& P, const 
   const float len = sqrtf(dot3(pc, pc));
   float d = 0.0f;
   v3f cp = make_v3(0.0f, 0.0f, 0.0f);
-  distmap_query(m, pos, d, cp, nullptr);
-  if (d < r) {
+  const bool near = !distmap_far(m, pos, r);
+  if (near) distmap_query(m, pos, d, cp, nullptr);
+  if (near && d < r) {
     const v3f n = normalize_ref(make_v3(pos.x - cp.x, pos.y - cp.y, pos.z - cp.z));
     const float pen = r - d;
     const float s2 = (float)(0.1 * (double)r);

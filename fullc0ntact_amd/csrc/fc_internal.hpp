@@ -230,6 +230,11 @@ struct RowHists;  // row_resort.hpp
 hipError_t launch_integrate_distmap(hipStream_t s, const fc_sim_params& P, const DistMapView& m,
                                     float4* pos, float4* vel, float dt, uint32_t n);
 DistMapView distmap_view(const fc_distmap* m);
+// Per 8^3-cell block of m, its smallest and largest vertex distance (for
+// distmap_far), in a new device array, or *out = nullptr when m's lattice
+// coordinates are not the uniform bmin + i cellSize.
+fc_status distmap_block_bounds(const fc_distmap* m, hipStream_t s, float2** out, int* bx,
+                               int* by);
 // index: the sorted values, particle ids; or, with idOfSlot, the slots of
 // oldPos/oldVel holding each particle (whose ids idOfSlot gives, written to
 // indexOut).
@@ -415,6 +420,9 @@ struct fc_system {
   // Boundary of the integrate stage.
   fc_boundary boundary = FC_BOUNDARY_BOX;
   const fc_distmap* boundaryMap = nullptr;
+  // The map's block bounds (taken when it is set; maps do not change).
+  float2* boundaryBounds = nullptr;
+  int boundaryBx = 0, boundaryBy = 0;
   // Radii from pos.w (set by fc_system_set_polydisperse).
   int polydisperse = 0;
   // Row resort tables of multi-step runs (allocated by the first run that

@@ -384,7 +384,12 @@ fc::BoundaryArg fc_system::boundary_arg() const {
   fc::BoundaryArg b{};
   b.hasMap = (boundary == FC_BOUNDARY_DISTMAP && boundaryMap != nullptr) ? 1 : 0;
   b.poly = polydisperse ? 1 : 0;
-  if (b.hasMap) b.map = distmap_view(boundaryMap);
+  if (b.hasMap) {
+    b.map = distmap_view(boundaryMap);
+    b.map.blockBounds = boundaryBounds;
+    b.map.bx = boundaryBx;
+    b.map.by = boundaryBy;
+  }
   return b;
 }
 
@@ -397,6 +402,14 @@ fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_dist
   if (mode == FC_BOUNDARY_DISTMAP && !map)
     return set_error(FC_ERROR_INVALID_VALUE, "the distance-map boundary needs a map");
   FC_HIP_TRY(hipStreamSynchronize(sys->ctx->stream));
+  float2* bounds = nullptr;
+  int bx = 0, by = 0;
+  if (mode == FC_BOUNDARY_DISTMAP)
+    FC_TRY(distmap_block_bounds(map, sys->ctx->stream, &bounds, &bx, &by));
+  (void)hipFree(sys->boundaryBounds);
+  sys->boundaryBounds = bounds;
+  sys->boundaryBx = bx;
+  sys->boundaryBy = by;
   sys->boundary = mode;
   sys->boundaryMap = mode == FC_BOUNDARY_DISTMAP ? map : nullptr;
   drop_graph(sys);  // the captured step carries the boundary in its arguments
@@ -460,6 +473,7 @@ fc_status fc_system_destroy(fc_system* sys) {
   // waits for the device before releasing memory.
   drop_graph(sys);
   free_system_buffers(sys);
+  (void)hipFree(sys->boundaryBounds);
   delete sys;
   return FC_SUCCESS;
 }
