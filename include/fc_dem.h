@@ -260,7 +260,11 @@ fc_status fc_system_device_arrays(fc_system* sys, float** pos, float** vel, floa
 fc_status fc_system_dump_grid(fc_system* sys, uint32_t* maxParticlesPerCell, uint32_t* occupiedCells);
 
 /* Boundary of the integrate stage (default FC_BOUNDARY_BOX).  The map must
- * outlive its use by the system. */
+ * outlive its use by the system.  Setting a map synchronises the context's
+ * stream and, for a map on the uniform lattice bmin + i cellSize, allocates
+ * the bounds of its 8^3-cell blocks (8 bytes per block), with which the
+ * integrate skips the query of spheres that cannot touch the boundary (the
+ * results are those of querying every sphere). */
 fc_status fc_system_set_boundary(fc_system* sys, fc_boundary mode, const fc_distmap* map);
 
 /* Polydisperse spheres.  No counterpart in the reference, which writes
