@@ -151,7 +151,7 @@ def stage_bytes(name: str, n: int, c_occ: int, c_touch: int, path: str = "run") 
     return None
 
 
-def survey_bytes(name: str, n: int, c_occ: int, c_touch: int) -> int | None:
+def reference_stage_bytes(name: str, n: int, c_occ: int, c_touch: int) -> int | None:
     """Algorithmic bytes of one launch by the reference-stage accounting:
     integrate 64 N, calcHash 24 N, sort 52 N (one histogram read + three
     passes of 8 B read and 8 B written), reorder 72 N + 16 C_occ, collide
@@ -597,6 +597,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-hopper", action="store_true", help="skip the polydisperse hopper leg")
+    ap.add_argument("--hopper-only", action="store_true",
+                    help="run only the polydisperse hopper leg and print its result")
     ap.add_argument("--hopper-settle", type=int, default=10)
     ap.add_argument("--hopper-steps", type=int, default=20)
     ap.add_argument("--profile-leg", choices=["dem", "sdf", "batch"], default="dem",
@@ -616,6 +618,10 @@ def main(argv=None) -> int:
     if a.small_case_only:
         print(json.dumps({"small_case": small_case_leg(host_threads()[0])}), flush=True)
         return 0
+    if a.hopper_only:
+        print(json.dumps({"polydisperse_hopper": hopper_leg(a.hopper_settle, a.hopper_steps)}),
+              flush=True)
+        return 0
 
     g = gpu_leg(a.steps, a.warmup, a.timing_steps, a.settle, a.progress_every)
     ms = g["elapsed"] / a.steps * 1e3
@@ -633,7 +639,7 @@ def main(argv=None) -> int:
 
     # The dominant kernel of the timed path: the largest total event time.
     dom = max(g["run_stages"].items(), key=lambda kv: kv[1][1])[0]
-    nbytes = survey_bytes(dom, N, g["c_occ"], g["c_touch"])
+    nbytes = reference_stage_bytes(dom, N, g["c_occ"], g["c_touch"])
     impl_bytes = stage_bytes(dom, N, g["c_occ"], g["c_touch"])
     roof = None
     if nbytes:
