@@ -318,11 +318,11 @@ def test_run_graph_equals_repeated_update(gpu_ctx):
 
 
 SMALL_GRIDS = {
-    # the single-block sort takes 9-bit digits over the width of numCells - 1:
-    # 8^3 = 9 bits (one pass), 32^3 = 15 bits (two), 64^3 = 18 bits (two),
-    # 128^3 = 21 bits (three), 64 x 64 x 16 = 16 bits (two, non-cubic).
-    # The cell stays 2r = 1/32, so the smaller grids wrap: many spheres of
-    # distant cells share a hash cell.
+    # key widths (of numCells - 1) on both sides of the small sort's 32-bit
+    # (key << 12) | index pairs, which hold keys of up to 20 bits: 8^3 = 9
+    # bits, 32^3 = 15, 64^3 = 18, 64 x 64 x 16 = 16 (non-cubic) and 128^3 =
+    # 21 (64-bit pairs).  The cell stays 2r = 1/32, so the smaller grids
+    # wrap: many spheres of distant cells share a hash cell.
     "8^3": (8, 8, 8),
     "32^3": (32, 32, 32),
     "64^3": (64, 64, 64),
