@@ -337,8 +337,8 @@ SMALL_CASES = [(n, "64^3") for n in (1, 2, 63, 65, 1000, 4095, 4096, 4097)] + [
 
 @pytest.mark.parametrize("n,grid_name", SMALL_CASES)
 def test_small_systems_run_and_update_match_oracle(gpu_ctx, n, grid_name):
-    # up to one sort tile (4096) a step sorts in one block that also writes
-    # the cell tables; 4097 takes the multi-block sort
+    # up to one sort tile (4096) a step sorts by counting (two launches that
+    # also write the cell tables); 4097 takes the multi-block sort
     from fullc0ntact_amd import ParticleSystem
     gs, r = SMALL_GRIDS[grid_name], 1.0 / 64.0
     print(f"[small] n={n} grid={grid_name}", flush=True)
