@@ -308,10 +308,32 @@ __global__ __launch_bounds__(256) void k_row_changers(RowTables T, const uint32_
       }
     }
   }
-  uint32_t at[kChgTiles];
+  // one atomic per bucket and tile: the lowest lane of each bucket's lanes
+  // (its leader) adds their count, every lane takes the leader's result
+  // plus its rank among them; the leaders of both tiles add in one go
+  uint32_t leader[kChgTiles], rank[kChgTiles], cnt[kChgTiles], at[kChgTiles];
+#pragma unroll
+  for (uint32_t i = 0; i < kChgTiles; ++i) {
+    leader[i] = lane;
+    rank[i] = cnt[i] = 0u;
+    uint64_t pending = __ballot(chg[i]);
+    while (pending != 0ull) {
+      const uint32_t l = lowest_lane(pending);
+      const uint32_t bl = __builtin_amdgcn_readlane(bucket[i], l);
+      const uint64_t peers = __ballot(chg[i] && bucket[i] == bl);
+      if ((peers >> lane) & 1ull) {
+        leader[i] = l;
+        rank[i] = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+      }
+      if (lane == l) cnt[i] = (uint32_t)__popcll(peers);
+      pending &= ~peers;
+    }
+  }
 #pragma unroll
   for (uint32_t i = 0; i < kChgTiles; ++i)
-    at[i] = chg[i] ? atomicAdd(&T.chgCursor[bucket[i]], 1u) : 0u;
+    at[i] = (chg[i] && leader[i] == lane) ? atomicAdd(&T.chgCursor[bucket[i]], cnt[i]) : 0u;
+#pragma unroll
+  for (uint32_t i = 0; i < kChgTiles; ++i) at[i] = __shfl(at[i], (int)leader[i]) + rank[i];
 #pragma unroll
   for (uint32_t i = 0; i < kChgTiles; ++i) {
     if (!chg[i]) continue;
